@@ -1,0 +1,130 @@
+/*
+ * pgw_oracle.h -- CPU restatement of the reference's block Wiedemann sequence
+ * path, used ONLY as a checker by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  Nothing in the product (the HIP library, its
+ * C ABI, the CLI) links, loads or calls this code.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository wilthoma/prym_green_ancillary).  Arithmetic is
+ * exact: u64/u128 sums and one reduction per output, generalised from the
+ * reference's uint16 storage to residues mod any prime p < 2^31.
+ *
+ * Parity pins (what checks each function, see tests/):
+ *   - subsets/incidences: brute force over itertools.combinations order and
+ *     the (-1)^position closed form (tests/test_oracle_subsets.py);
+ *   - sweeps / eliminated operator: dense brute-force assembly on tiny inputs,
+ *     transpose identities, and the genus-12 certificate K0 (F0*K0 == 0),
+ *     F^(0) nnz == 8400 and the sector ranks of g12-reference.json;
+ *   - splitmix64 / FNV-1a / mt19937_64: published test vectors;
+ *   - uniform_int (libstdc++ 11 Lemire draw): golden vectors written by g++;
+ *   - Gram / sequence: closed form V^T S^t V against dense powers;
+ *   - WDM text: the hand-made diag(1,2) file of the BCW CLI test.
+ */
+#ifndef PGW_ORACLE_H
+#define PGW_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- combinatorics (cuda/cuprym_cyclic.cu:134-230) ---- */
+uint64_t orc_binom(uint32_t n, uint32_t k);
+int orc_unrank_subset(uint64_t rank, uint32_t n, uint32_t k, uint32_t* out);
+
+/* ---- RNG and hashes (cuda/cuprym_cyclic.cu:1143-1218, cuda/matrices.h:44-56) ---- */
+uint64_t orc_splitmix64_next(uint64_t* state);
+uint64_t orc_seeded_sector_value(uint64_t seed, uint64_t sector);
+uint64_t orc_rowmix_bounded(uint64_t* state, uint64_t bound);
+uint64_t orc_fnv1a64_u32_vector(const uint32_t* values, size_t count);
+uint64_t orc_fnv1a64_string(const char* s);
+/* mt19937_64 seeded with `seed`, first `count` raw outputs. */
+void orc_mt19937_64_outputs(uint64_t seed, size_t count, uint64_t* out);
+/* Draw order of run_eliminated_sector (cuda/cuprym_cyclic.cu:2196-2199):
+ * Dr (nonzero, rows), Dc (nonzero, cols), V (cols*b, row-major). */
+void orc_reference_random_state(uint64_t seed, uint64_t sector, uint32_t p,
+                                size_t rows, size_t cols, size_t b,
+                                uint32_t* dr, uint32_t* dc, uint32_t* v);
+/* Uniform draws in [lo, hi] from mt19937_64(seed) (libstdc++ 11 semantics). */
+void orc_uniform_u64(uint64_t seed, uint64_t lo, uint64_t hi, size_t count, uint64_t* out);
+
+/* ---- row mixing (cuda/cuprym_cyclic.cu:1196-1287) ---- */
+uint64_t orc_rowmix_realized_seed(uint64_t seed, const char* op, uint64_t sector,
+                                  uint64_t rows, uint32_t p);
+/* pairs: rounds * (rows/2) records of {first, second, a, b}. */
+void orc_rowmix_plan(uint64_t realized_seed, size_t rows, size_t rounds, uint32_t p,
+                     uint32_t* pairs);
+uint64_t orc_rowmix_plan_hash(size_t rows, size_t rounds, uint64_t seed,
+                              uint64_t realized_seed, const uint32_t* pairs);
+
+/* ---- synthetic tables (uniform in [0,p) from a splitmix64 stream) ---- */
+void orc_synthetic_table(uint64_t seed, uint32_t p, size_t count, uint32_t* out);
+
+/* ---- operators ---- */
+typedef struct orc_op orc_op;
+
+/* Synthetic Koszul pair: D uses table T[i][beta][a] (forward gather),
+ * E uses table E[i][beta][a] (transpose gather); F = E o D, square N x N,
+ * N = C(n,k)*M, mid R = C(n,k-1)*Mp. */
+orc_op* orc_synthetic_create(uint32_t n, uint32_t k, uint32_t M, uint32_t Mp, uint32_t p,
+                             const uint32_t* T, const uint32_t* E,
+                             const uint32_t* dr, const uint32_t* dc,
+                             size_t rowmix_rounds, const uint32_t* rowmix_pairs);
+
+/* Reference eliminated cyclic operator (cuda/cuprym_cyclic.cu:2136-2418). */
+orc_op* orc_cyclic_create(uint32_t genus, uint32_t r, uint32_t p, uint32_t sector,
+                          const uint32_t* mu, const uint16_t* weight_v,
+                          const uint16_t* weight_a0, const uint16_t* weight_a1,
+                          uint32_t w_index, uint32_t w_weight, uint32_t kernel_dim,
+                          const uint16_t* kernel_weights, const uint32_t* right_inverse,
+                          const uint32_t* kernel_basis);
+/* Set preconditioners / rowmix after the dims are known (cyclic op). */
+void orc_set_preconditioners(orc_op* op, const uint32_t* dr, const uint32_t* dc,
+                             size_t rowmix_rounds, const uint32_t* rowmix_pairs);
+void orc_op_dims(const orc_op* op, size_t* rows, size_t* cols);
+void orc_op_destroy(orc_op* op);
+
+/* F*(Dc*X) without row preconditioning (certificate product, cuprym_cyclic.cu:2330-2359). */
+void orc_apply_forward(orc_op* op, const uint32_t* in, uint32_t* out, size_t b);
+/* Dc*F^T*X (cuprym_cyclic.cu:2361-2395). */
+void orc_apply_backward(orc_op* op, const uint32_t* in, uint32_t* out, size_t b);
+/* S*X = Dc F^T P^T Dr P F Dc X (cuprym_cyclic.cu:2408-2418, 1939-1966). */
+void orc_apply_square(orc_op* op, const uint32_t* in, uint32_t* out, size_t b);
+/* One Koszul sweep of a synthetic operator restricted to output rows
+ * [row0,row1): which = 0 forward gather with table T (the D sweep),
+ * 1 transpose gather with table E (the E sweep).  Used only to time a bounded
+ * CPU sample of the apply; `out` must hold the whole sweep output. */
+void orc_sweep_partial(orc_op* op, int which, const uint32_t* in, uint32_t* out, size_t b,
+                       size_t row0, size_t row1);
+/* Output rows of the two synthetic sweeps (R for the D sweep, N for E). */
+size_t orc_sweep_rows(const orc_op* op, int which);
+
+/* upper-triangle Gram, packed seq[ut(i,j)] (cuda/prym_cuda_helpers.h:87-137,318-354) */
+void orc_gram_upper(const uint32_t* x, const uint32_t* y, size_t rows, size_t b,
+                    uint32_t p, uint32_t* out);
+/* Full sequence loop (cuprym_cyclic.cu:2326-2450): seq is L x b(b+1)/2
+ * with L = 1 + 4*ceil((max_len-1)/4); curv is S^(L-1) V. Returns L. */
+size_t orc_sequence(orc_op* op, const uint32_t* v, size_t b, size_t max_len,
+                    uint32_t* seq, uint32_t* curv);
+/* D[r][c] += sum_i B[r][i]*G[c*b+i] (cuprym_cyclic_kernel_vectors.cu:1766-1790). */
+void orc_accumulate_generator(const uint32_t* B, const uint32_t* G, uint32_t* D,
+                              size_t rows, size_t b, uint32_t p);
+/* Kernel-vector recovery core (cuprym_cyclic_kernel_vectors.cu:2549-2628):
+ * D = sum_k S^k V G_k, product = F*(Dc*D).  */
+void orc_kernel_vectors(orc_op* op, const uint32_t* v, size_t b, const uint32_t* gens,
+                        size_t ngen, uint32_t* d_out, uint32_t* product_out);
+/* Rank of a dense rows x cols matrix mod p (cuprym_cyclic_kernel_vectors.cu:1490-1546). */
+size_t orc_rank_mod_p(const uint32_t* m, size_t rows, size_t cols, uint32_t p,
+                      uint32_t* pivot_cols);
+
+/* WDM text writer (cuda/wdmfiles.h:153-222), uncompressed. seq is L x b(b+1)/2. */
+int orc_write_wdm(const char* path, size_t rows, size_t cols, uint32_t p,
+                  const uint32_t* dr, const uint32_t* dc, const uint32_t* v,
+                  const uint32_t* curv, size_t b, const uint32_t* seq, size_t L);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
