@@ -1,0 +1,200 @@
+/*
+ * pgw.h -- C ABI of the MI355X block Wiedemann sequence generator (libpgw.so).
+ *
+ * The reference (wilthoma/prym_green_ancillary) has no FFI: its boundary is the
+ * `cuprym_cyclic` / `cuprym_cyclic_kernel_vectors` executables plus the
+ * header-only C++ operator API compiled into them.  Each entry point below
+ * names the reference interface it replaces.  Conventions:
+ *   - every call returns PGW_OK (0) or a nonzero pgw_status; no exception ever
+ *     crosses the ABI; pgw_last_error() gives the message (thread-local);
+ *   - blocks are row-major N x b arrays of uint32 canonical residues
+ *     (X[row*b + col]), exactly the reference's dense layout
+ *     (cuda/cyclic_cuda_helpers.h:185, X[(colbase+a)*DENSE_COLS + c]);
+ *   - d_* pointers are device memory owned by the caller, h_* host memory;
+ *   - `stream` is a hipStream_t (NULL = the context's own stream); calls on
+ *     one operator are serialised on the stream they are given;
+ *   - an operator is thread-compatible, not thread-safe.
+ */
+#ifndef PGW_H
+#define PGW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pgw_ctx pgw_ctx;
+typedef struct pgw_op pgw_op;
+
+enum pgw_status {
+    PGW_OK = 0,
+    PGW_ERR_INVALID = 1,      /* bad argument or shape */
+    PGW_ERR_HIP = 2,          /* HIP runtime / launch failure */
+    PGW_ERR_MEMORY = 3,       /* device or host allocation failed */
+    PGW_ERR_UNSUPPORTED = 4,  /* valid request outside what this build runs */
+    PGW_ERR_IO = 5            /* file I/O (WDM writer / reader) */
+};
+
+enum pgw_kind {
+    /* A = E o D over Lambda^k V (x) R_d, dense multiplication tables (r = 1). */
+    PGW_KIND_SYNTHETIC = 1,
+    /* the reference's eliminated cyclic-sector operator
+     * F = [ -D_(m-1) R D_m | D_(m-1) B ]  (cuda/cuprym_cyclic.cu:2330-2359) */
+    PGW_KIND_CYCLIC_ELIMINATED = 2
+};
+
+enum pgw_method {
+    PGW_METHOD_AUTO = 0,   /* MFMA where the operator allows it, else plain */
+    PGW_METHOD_PLAIN = 1,  /* u64 modmul gathers on the vector ALUs */
+    PGW_METHOD_MFMA = 2    /* int8-limb MFMA table contraction (synthetic only) */
+};
+
+/* One 2x2 row-mixing map [[1,a],[b,1+ab]] on target rows (first, second).
+ * Replaces CyclicRowMixPair (cuda/cyclic_cuda_helpers.h:39-44), widened to
+ * u32 coefficients so that p may exceed 65535. */
+typedef struct {
+    uint32_t first;
+    uint32_t second;
+    uint32_t a;
+    uint32_t b;
+} pgw_rowmix_pair;
+
+/* Operator description; arrays are deep-copied by pgw_operator_create.
+ * Replaces CyclicPhiHostData + the elimination arrays of run_eliminated_sector
+ * (cuda/cyclic_cuda_helpers.h:46-75, cuda/cuprym_cyclic.cu:2136-2298). */
+typedef struct {
+    uint32_t kind;        /* pgw_kind */
+    uint32_t prime;       /* p, prime, 2 < p < 2^31 */
+    uint32_t block;       /* b, block width, 1..16 */
+    uint32_t method;      /* pgw_method */
+    /* PGW_KIND_SYNTHETIC: D: Lambda^k (x) R_d -> Lambda^(k-1) (x) R_(d+1) with
+     * table_d[i][beta][a]; E: Lambda^(k-1) (x) R_(d+1) -> Lambda^k (x) R_d with
+     * table_e[i][beta][a]; i < n, beta < m_tgt, a < m_src. */
+    uint32_t n;
+    uint32_t k;
+    uint32_t m_src;       /* M  = dim R_d   */
+    uint32_t m_tgt;       /* M' = dim R_(d+1) */
+    const uint32_t* table_d;
+    const uint32_t* table_e;
+    /* PGW_KIND_CYCLIC_ELIMINATED: the instance JSON fields
+     * (cuda/cuprym_cyclic.cu:483-609); n = genus - 3. */
+    uint32_t genus;
+    uint32_t order;       /* r, cyclic order */
+    uint32_t sector;
+    const uint32_t* mu;   /* [n][n][genus], layout ((i*n)+beta)*genus+a */
+    const uint16_t* weight_v;
+    const uint16_t* weight_a0;
+    const uint16_t* weight_a1;
+    uint32_t w_index;
+    uint32_t w_weight;
+    uint32_t kernel_dim;
+    const uint16_t* kernel_weights;
+    const uint32_t* right_inverse;   /* [genus][n]          */
+    const uint32_t* kernel_basis;    /* [genus][kernel_dim] */
+} pgw_operator_desc;
+
+typedef struct {
+    uint64_t rows;          /* target rows of F */
+    uint64_t cols;          /* source columns of F = N, the block height */
+    uint64_t mid_rows;      /* intermediate space (R for synthetic, y for cyclic) */
+    uint64_t device_bytes;  /* device memory the operator holds */
+    uint32_t method;        /* resolved pgw_method */
+    uint32_t block;
+} pgw_op_info;
+
+/* ---- errors and contexts ------------------------------------------------ */
+
+/* Message of the last failed call on this thread (never NULL). */
+const char* pgw_last_error(void);
+/* Replaces cudaSetDevice + the legacy default stream (cuprym_cyclic.cu:2578). */
+int pgw_ctx_create(int device, pgw_ctx** out);
+int pgw_ctx_destroy(pgw_ctx* ctx);
+int pgw_ctx_synchronize(pgw_ctx* ctx);
+/* Device name and total memory (cuprym_cyclic.cu:2579-2583). */
+int pgw_ctx_device_info(pgw_ctx* ctx, char* name, size_t name_len, uint64_t* total_bytes);
+
+/* ---- host-side recipe helpers (same bits as the reference driver) -------- */
+
+/* seeded_sector_value (cuda/cuprym_cyclic.cu:1143-1151). */
+uint64_t pgw_seeded_sector_value(uint64_t seed, uint64_t sector);
+/* mt19937_64(seeded_sector_value(seed, sector)) draws Dr (rows, nonzero),
+ * Dc (cols, nonzero), V (cols*b) in that order (cuda/cuprym_cyclic.cu:2196-2199,
+ * cuda/matrices.h:44-56). */
+int pgw_reference_random_state(uint64_t seed, uint64_t sector, uint32_t prime, uint64_t rows,
+                               uint64_t cols, uint32_t block, uint32_t* h_dr, uint32_t* h_dc,
+                               uint32_t* h_v);
+/* rowmix_realized_seed (cuda/cuprym_cyclic.cu:1196-1205). */
+uint64_t pgw_rowmix_realized_seed(uint64_t seed, const char* op_name, uint64_t sector,
+                                  uint64_t rows, uint32_t prime);
+/* generate_rowmix_plan (cuda/cuprym_cyclic.cu:1227-1268): rounds*(rows/2) pairs. */
+int pgw_rowmix_plan(uint64_t realized_seed, uint64_t rows, uint32_t rounds, uint32_t prime,
+                    pgw_rowmix_pair* h_pairs);
+/* rowmix_plan_hash (cuda/cuprym_cyclic.cu:1270-1287). */
+uint64_t pgw_rowmix_plan_hash(uint64_t rows, uint32_t rounds, uint64_t seed,
+                              uint64_t realized_seed, const pgw_rowmix_pair* h_pairs);
+/* fnv1a64_vector_hash over u64 little-endian (cuda/cuprym_cyclic.cu:1181-1188). */
+uint64_t pgw_fnv1a64_u32(const uint32_t* h_values, uint64_t count);
+/* Synthetic multiplication table: `count` values uniform in [0,p) from a
+ * splitmix64 stream with rejection-bounded draws (rowmix_bounded,
+ * cuda/cuprym_cyclic.cu:1207-1218). No counterpart table exists in the
+ * reference; the draw procedure is the reference's. */
+int pgw_synthetic_table(uint64_t seed, uint32_t prime, uint64_t count, uint32_t* h_out);
+
+/* ---- operators ------------------------------------------------------------ */
+
+/* Shape of the operator a description defines, without touching the GPU
+ * (build_sector_layout_generic / build_tensor_layout sizes,
+ * cuda/cuprym_cyclic.cu:2161-2176). */
+int pgw_operator_dims(const pgw_operator_desc* desc, uint64_t* rows, uint64_t* cols);
+/* Replaces CudaCyclicPhiOperator::from_host x2 + the device copies of the
+ * elimination data, preconditioners and row-mix plan (cuprym_cyclic.cu:2240-2324,
+ * cuda/cyclic_cuda_helpers.h:297-360).  h_row_precond has `rows` entries (Dr),
+ * h_col_precond `cols` entries (Dc), h_pairs rowmix_rounds*(rows/2) entries. */
+int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
+                        const uint32_t* h_row_precond, const uint32_t* h_col_precond,
+                        uint32_t rowmix_rounds, const pgw_rowmix_pair* h_pairs, pgw_op** out);
+/* Replaces CudaCyclicPhiOperator::release (cuda/cyclic_cuda_helpers.h:362-394). */
+int pgw_operator_destroy(pgw_op* op);
+int pgw_operator_info(const pgw_op* op, pgw_op_info* info);
+
+/* One block-apply d_out = S*d_in, S = Dc F^T P^T Dr P F Dc, both N x b.
+ * Replaces the apply_square lambda (cuda/cuprym_cyclic.cu:2408-2418). */
+int pgw_apply_square(pgw_op* op, const uint32_t* d_in, uint32_t* d_out, void* stream);
+/* d_out (rows x b) = F*(Dc*d_in): the certificate product
+ * (apply_forward, cuda/cuprym_cyclic.cu:2330-2359; kernel_vectors.cu:2628). */
+int pgw_apply_forward(pgw_op* op, const uint32_t* d_in, uint32_t* d_out, void* stream);
+/* Packed upper triangle of d_x^T d_y mod p into d_out[b(b+1)/2], WDM order
+ * ut(i,j) = i(2b-i+1)/2 + (j-i).  Replaces mTm_tri_pack_from_scratch
+ * (cuda/prym_cuda_helpers.h:318-354) without its u32 atomic accumulators. */
+int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_t* d_out,
+                   void* stream);
+
+/* Progress callback: terms done, terms total. */
+typedef void (*pgw_progress_fn)(uint64_t done, uint64_t total, void* user);
+
+/* Whole sequence loop (cuprym_cyclic.cu:2316-2450): seq[0] = V^T V, then
+ * ceil((max_len-1)/4) rounds of {D=S B; B^T D; D^T D; B=S D; B^T D; B^T B}.
+ * h_seq receives L x b(b+1)/2 values with L = 1+4*ceil((max_len-1)/4)
+ * (row t = packed V^T S^t V), h_curv the final N x b block.  *out_len = L. */
+int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t* h_seq,
+                     uint32_t* h_curv, uint64_t* out_len, pgw_progress_fn progress,
+                     void* user);
+
+/* Kernel-vector recovery core (cuprym_cyclic_kernel_vectors.cu:2549-2628):
+ * h_d = sum_{k<ngen} S^k V G_k (G_k[c*b+i] multiplies column i into c),
+ * h_product = F*(Dc*h_d) (rows x b, all zero for a certificate). */
+int pgw_kernel_vectors(pgw_op* op, const uint32_t* h_v, const uint32_t* h_gens, uint64_t ngen,
+                       uint32_t* h_d, uint32_t* h_product);
+
+/* Timing of the hot path on the operator's own stream with HIP events:
+ * `iters` block-applies d_out = S d_in (ping-pong), total milliseconds and
+ * the per-launch average of the dominant sweep kernel (ms). */
+int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters,
+                    double* total_ms, double* sweep_avg_ms, uint32_t* sweeps_per_apply);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

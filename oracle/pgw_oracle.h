@@ -105,7 +105,8 @@ size_t orc_sweep_rows(const orc_op* op, int which);
 void orc_gram_upper(const uint32_t* x, const uint32_t* y, size_t rows, size_t b,
                     uint32_t p, uint32_t* out);
 /* Full sequence loop (cuprym_cyclic.cu:2326-2450): seq is L x b(b+1)/2
- * with L = 1 + 4*ceil((max_len-1)/4); curv is S^(L-1) V. Returns L. */
+ * with L = 1 + 4*ceil((max_len-1)/4); curv is S^(2*rounds) V (two applies per
+ * round of four terms). Returns L. */
 size_t orc_sequence(orc_op* op, const uint32_t* v, size_t b, size_t max_len,
                     uint32_t* seq, uint32_t* curv);
 /* D[r][c] += sum_i B[r][i]*G[c*b+i] (cuprym_cyclic_kernel_vectors.cu:1766-1790). */

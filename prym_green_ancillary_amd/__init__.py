@@ -1,0 +1,391 @@
+"""MI355X block Wiedemann sequence generator -- Python host interface.
+
+A thin ctypes layer over libpgw.so (include/pgw.h).  It mirrors the host side
+of the reference's `cuprym_cyclic` driver (cuda/cuprym_cyclic.cu): the
+operator of one sector, its preconditioners and row-mixing plan, the block
+apply S = Dc F^T P^T Dr P F Dc, the packed Gram projection and the whole
+sequence loop.  All arithmetic runs in the HIP library; this module never
+computes a residue itself, and it raises if the native library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "NativeLibraryError", "PgwError", "lib", "library_path", "Context", "Operator",
+    "reference_random_state", "rowmix_realized_seed", "rowmix_plan", "rowmix_plan_hash",
+    "fnv1a64_u32", "synthetic_table", "SyntheticSpec", "synthetic_spec_from_json",
+    "load_cyclic_instance", "METHOD_AUTO", "METHOD_PLAIN", "METHOD_MFMA", "sequence_length",
+]
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libpgw.so"
+
+KIND_SYNTHETIC = 1
+KIND_CYCLIC = 2
+METHOD_AUTO, METHOD_PLAIN, METHOD_MFMA = 0, 1, 2
+
+
+class NativeLibraryError(RuntimeError):
+    """libpgw.so is missing or does not load: there is no fallback path."""
+
+
+class PgwError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"pgw error {status}: {message}")
+        self.status = status
+
+
+class RowMixPair(ctypes.Structure):
+    _fields_ = [("first", ctypes.c_uint32), ("second", ctypes.c_uint32),
+                ("a", ctypes.c_uint32), ("b", ctypes.c_uint32)]
+
+
+class OperatorDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_uint32), ("prime", ctypes.c_uint32), ("block", ctypes.c_uint32),
+        ("method", ctypes.c_uint32),
+        ("n", ctypes.c_uint32), ("k", ctypes.c_uint32), ("m_src", ctypes.c_uint32),
+        ("m_tgt", ctypes.c_uint32),
+        ("table_d", ctypes.c_void_p), ("table_e", ctypes.c_void_p),
+        ("genus", ctypes.c_uint32), ("order", ctypes.c_uint32), ("sector", ctypes.c_uint32),
+        ("mu", ctypes.c_void_p), ("weight_v", ctypes.c_void_p), ("weight_a0", ctypes.c_void_p),
+        ("weight_a1", ctypes.c_void_p),
+        ("w_index", ctypes.c_uint32), ("w_weight", ctypes.c_uint32), ("kernel_dim", ctypes.c_uint32),
+        ("kernel_weights", ctypes.c_void_p), ("right_inverse", ctypes.c_void_p),
+        ("kernel_basis", ctypes.c_void_p),
+    ]
+
+
+class OpInfo(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_uint64), ("cols", ctypes.c_uint64), ("mid_rows", ctypes.c_uint64),
+                ("device_bytes", ctypes.c_uint64), ("method", ctypes.c_uint32),
+                ("block", ctypes.c_uint32)]
+
+
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p)
+
+_lib = None
+
+# every symbol include/pgw.h declares (checked by the CPU test suite)
+EXPORTED = [
+    "pgw_last_error", "pgw_ctx_create", "pgw_ctx_destroy", "pgw_ctx_synchronize",
+    "pgw_ctx_device_info", "pgw_seeded_sector_value", "pgw_reference_random_state",
+    "pgw_rowmix_realized_seed", "pgw_rowmix_plan", "pgw_rowmix_plan_hash", "pgw_fnv1a64_u32",
+    "pgw_synthetic_table", "pgw_operator_dims", "pgw_operator_create", "pgw_operator_destroy",
+    "pgw_operator_info", "pgw_apply_square", "pgw_apply_forward", "pgw_gram_upper",
+    "pgw_sequence_run", "pgw_kernel_vectors", "pgw_bench_apply",
+]
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("PGW_LIBRARY", LIB_PATH))
+
+
+def lib():
+    """Load libpgw.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not path.exists():
+        raise NativeLibraryError(f"{path} is missing: build it with __graft_entry__.build() "
+                                 "(make -C prym_green_ancillary_amd/csrc)")
+    try:
+        L = ctypes.CDLL(str(path))
+    except OSError as e:  # pragma: no cover - loader failure
+        raise NativeLibraryError(f"cannot load {path}: {e}") from e
+    u32, u64, vp, sz = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_size_t
+    I = ctypes.c_int
+    sig = {
+        "pgw_last_error": (ctypes.c_char_p, []),
+        "pgw_ctx_create": (I, [I, ctypes.POINTER(vp)]),
+        "pgw_ctx_destroy": (I, [vp]),
+        "pgw_ctx_synchronize": (I, [vp]),
+        "pgw_ctx_device_info": (I, [vp, ctypes.c_char_p, sz, ctypes.POINTER(u64)]),
+        "pgw_seeded_sector_value": (u64, [u64, u64]),
+        "pgw_reference_random_state": (I, [u64, u64, u32, u64, u64, u32, vp, vp, vp]),
+        "pgw_rowmix_realized_seed": (u64, [u64, ctypes.c_char_p, u64, u64, u32]),
+        "pgw_rowmix_plan": (I, [u64, u64, u32, u32, vp]),
+        "pgw_rowmix_plan_hash": (u64, [u64, u32, u64, u64, vp]),
+        "pgw_fnv1a64_u32": (u64, [vp, u64]),
+        "pgw_synthetic_table": (I, [u64, u32, u64, vp]),
+        "pgw_operator_dims": (I, [ctypes.POINTER(OperatorDesc), ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "pgw_operator_create": (I, [vp, ctypes.POINTER(OperatorDesc), vp, vp, u32, vp, ctypes.POINTER(vp)]),
+        "pgw_operator_destroy": (I, [vp]),
+        "pgw_operator_info": (I, [vp, ctypes.POINTER(OpInfo)]),
+        "pgw_apply_square": (I, [vp, vp, vp, vp]),
+        "pgw_apply_forward": (I, [vp, vp, vp, vp]),
+        "pgw_gram_upper": (I, [vp, vp, vp, vp, vp]),
+        "pgw_sequence_run": (I, [vp, vp, u64, vp, vp, ctypes.POINTER(u64), PROGRESS_FN, vp]),
+        "pgw_kernel_vectors": (I, [vp, vp, vp, u64, vp, vp]),
+        "pgw_bench_apply": (I, [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise PgwError(rc, lib().pgw_last_error().decode())
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# host recipe helpers (cuda/cuprym_cyclic.cu:1143-1287)
+# ---------------------------------------------------------------------------
+
+def reference_random_state(seed: int, sector: int, p: int, rows: int, cols: int, b: int):
+    """Dr, Dc, V exactly as run_eliminated_sector draws them (cuprym_cyclic.cu:2196-2199)."""
+    dr = np.empty(rows, np.uint32)
+    dc = np.empty(cols, np.uint32)
+    v = np.empty(cols * b, np.uint32)
+    _check(lib().pgw_reference_random_state(seed, sector, p, rows, cols, b, _ptr(dr), _ptr(dc), _ptr(v)))
+    return dr, dc, v
+
+
+def rowmix_realized_seed(seed: int, op_name: str, sector: int, rows: int, p: int) -> int:
+    return lib().pgw_rowmix_realized_seed(seed, op_name.encode(), sector, rows, p)
+
+
+def rowmix_plan(realized_seed: int, rows: int, rounds: int, p: int) -> np.ndarray:
+    """rounds*(rows//2) x 4 array of (first, second, a, b)."""
+    pairs = np.zeros((max(1, rounds * (rows // 2)), 4), np.uint32)
+    _check(lib().pgw_rowmix_plan(realized_seed, rows, rounds, p, _ptr(pairs)))
+    return pairs[: rounds * (rows // 2)]
+
+
+def rowmix_plan_hash(rows: int, rounds: int, seed: int, realized: int, pairs: np.ndarray) -> int:
+    pairs = _u32(pairs) if len(pairs) else np.zeros((1, 4), np.uint32)
+    return lib().pgw_rowmix_plan_hash(rows, rounds, seed, realized, _ptr(pairs))
+
+
+def fnv1a64_u32(values) -> int:
+    v = _u32(values)
+    return lib().pgw_fnv1a64_u32(_ptr(v) if v.size else None, v.size)
+
+
+def synthetic_table(seed: int, p: int, count: int) -> np.ndarray:
+    out = np.empty(count, np.uint32)
+    _check(lib().pgw_synthetic_table(seed, p, count, _ptr(out)))
+    return out
+
+
+def sequence_length(max_len: int) -> int:
+    """L = 1 + 4*ceil((max_len-1)/4) (cuprym_cyclic.cu:2425)."""
+    return 1 + 4 * ((max_len - 1 + 3) // 4)
+
+
+# ---------------------------------------------------------------------------
+# specs
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SyntheticSpec:
+    """A = E o D on Lambda^k V (x) R_d with tables drawn from two splitmix64 seeds."""
+    n: int
+    k: int
+    m_src: int
+    m_tgt: int
+    prime: int
+    table_seed_d: int
+    table_seed_e: int
+
+    def tables(self):
+        count = self.n * self.m_tgt * self.m_src
+        return (synthetic_table(self.table_seed_d, self.prime, count),
+                synthetic_table(self.table_seed_e, self.prime, count))
+
+    @property
+    def cols(self) -> int:
+        from math import comb
+        return comb(self.n, self.k) * self.m_src
+
+
+def synthetic_spec_from_json(obj: dict) -> SyntheticSpec:
+    s = obj["synthetic"]
+    return SyntheticSpec(int(s["n"]), int(s["k"]), int(s["M"]), int(s["M_prime"]),
+                         int(obj["modulus"]), int(s["table_seed_d"]), int(s["table_seed_e"]))
+
+
+def load_cyclic_instance(path) -> dict:
+    with open(path) as f:
+        return json.load(f)
+
+
+# ---------------------------------------------------------------------------
+# context and operator
+# ---------------------------------------------------------------------------
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().pgw_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        _check(lib().pgw_ctx_synchronize(self.h))
+
+    def device_info(self):
+        name = ctypes.create_string_buffer(256)
+        total = ctypes.c_uint64()
+        _check(lib().pgw_ctx_device_info(self.h, name, 256, ctypes.byref(total)))
+        return name.value.decode(), total.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgw_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Operator:
+    """One sector operator on the GPU (pgw_operator_create)."""
+
+    def __init__(self, ctx: Context, desc: OperatorDesc, dr, dc, rounds: int, pairs, keep):
+        self.ctx = ctx
+        dr, dc = _u32(dr), _u32(dc)
+        pairs_arr = _u32(pairs).reshape(-1, 4) if pairs is not None and len(pairs) else None
+        h = ctypes.c_void_p()
+        _check(lib().pgw_operator_create(ctx.h, ctypes.byref(desc), _ptr(dr), _ptr(dc), rounds,
+                                         _ptr(pairs_arr) if pairs_arr is not None else None,
+                                         ctypes.byref(h)))
+        self.h = h
+        info = OpInfo()
+        _check(lib().pgw_operator_info(self.h, ctypes.byref(info)))
+        self.rows, self.cols, self.mid = info.rows, info.cols, info.mid_rows
+        self.device_bytes = info.device_bytes
+        self.method = info.method
+        self.block = info.block
+        self.prime = desc.prime
+
+    # -- construction ------------------------------------------------------
+    @staticmethod
+    def synthetic_desc(spec: SyntheticSpec, block: int, method: int, table_d, table_e):
+        d = OperatorDesc()
+        d.kind, d.prime, d.block, d.method = KIND_SYNTHETIC, spec.prime, block, method
+        d.n, d.k, d.m_src, d.m_tgt = spec.n, spec.k, spec.m_src, spec.m_tgt
+        d.table_d, d.table_e = table_d.ctypes.data, table_e.ctypes.data
+        return d
+
+    @classmethod
+    def synthetic(cls, ctx, spec: SyntheticSpec, block: int, dr, dc, rounds=0, pairs=None,
+                  method=METHOD_AUTO, tables=None):
+        td, te = tables if tables is not None else spec.tables()
+        td, te = _u32(td), _u32(te)
+        d = cls.synthetic_desc(spec, block, method, td, te)
+        return cls(ctx, d, dr, dc, rounds, pairs, [td, te])
+
+    @staticmethod
+    def cyclic_desc(inst: dict, sector: int, block: int):
+        g, p, r = int(inst["genus"]), int(inst["modulus"]), int(inst["cyclic"]["order"])
+        w = inst["weights"]
+        e = inst["third_section_elimination"]
+        arrays = dict(
+            mu=_u32(np.asarray(inst["artinian"]["mu"]["values"]).reshape(-1)),
+            wv=np.ascontiguousarray(np.asarray(w["V"], np.uint16)),
+            wa0=np.ascontiguousarray(np.asarray(w["A0"], np.uint16)),
+            wa1=np.ascontiguousarray(np.asarray(w["A1"], np.uint16)),
+            kw=np.ascontiguousarray(np.asarray(e["kernel_weights"], np.uint16)),
+            R=_u32(np.asarray(e["right_inverse_A0_by_A1"]).reshape(-1)),
+            B=_u32(np.asarray(e["kernel_basis_A0_by_3"]).reshape(-1)),
+        )
+        d = OperatorDesc()
+        d.kind, d.prime, d.block, d.method = KIND_CYCLIC, p, block, METHOD_PLAIN
+        d.genus, d.order, d.sector = g, r, sector
+        d.mu = arrays["mu"].ctypes.data
+        d.weight_v, d.weight_a0, d.weight_a1 = (arrays["wv"].ctypes.data, arrays["wa0"].ctypes.data,
+                                                arrays["wa1"].ctypes.data)
+        d.w_index, d.w_weight = int(e["w_V_index"]), int(e["w_weight"])
+        d.kernel_dim = arrays["kw"].size
+        d.kernel_weights = arrays["kw"].ctypes.data
+        d.right_inverse, d.kernel_basis = arrays["R"].ctypes.data, arrays["B"].ctypes.data
+        return d, arrays
+
+    @classmethod
+    def cyclic(cls, ctx, inst: dict, sector: int, block: int, dr, dc, rounds=0, pairs=None):
+        d, keep = cls.cyclic_desc(inst, sector, block)
+        return cls(ctx, d, dr, dc, rounds, pairs, keep)
+
+    @staticmethod
+    def dims(desc: OperatorDesc):
+        rows, cols = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().pgw_operator_dims(ctypes.byref(desc), ctypes.byref(rows), ctypes.byref(cols)))
+        return rows.value, cols.value
+
+    # -- device-pointer entry points ------------------------------------------
+    def apply_square(self, d_in: int, d_out: int, stream: int | None = None):
+        _check(lib().pgw_apply_square(self.h, d_in, d_out, stream))
+
+    def apply_forward(self, d_in: int, d_out: int, stream: int | None = None):
+        _check(lib().pgw_apply_forward(self.h, d_in, d_out, stream))
+
+    def gram_upper(self, d_x: int, d_y: int, d_out: int, stream: int | None = None):
+        _check(lib().pgw_gram_upper(self.h, d_x, d_y, d_out, stream))
+
+    def bench_apply(self, d_a: int, d_b: int, iters: int):
+        total, sweep = ctypes.c_double(), ctypes.c_double()
+        count = ctypes.c_uint32()
+        _check(lib().pgw_bench_apply(self.h, d_a, d_b, iters, ctypes.byref(total),
+                                     ctypes.byref(sweep), ctypes.byref(count)))
+        return total.value, sweep.value, count.value
+
+    # -- host-buffer entry points ------------------------------------------------
+    def sequence(self, v, max_len: int, progress=None):
+        v = _u32(v).reshape(-1)
+        b = self.block
+        ut = b * (b + 1) // 2
+        L = sequence_length(max_len)
+        seq = np.empty(L * ut, np.uint32)
+        curv = np.empty(self.cols * b, np.uint32)
+        out_len = ctypes.c_uint64()
+        cb = PROGRESS_FN(lambda done, total, user: progress(done, total)) if progress else PROGRESS_FN()
+        _check(lib().pgw_sequence_run(self.h, _ptr(v), max_len, _ptr(seq), _ptr(curv),
+                                      ctypes.byref(out_len), cb, None))
+        assert out_len.value == L
+        return seq.reshape(L, ut), curv
+
+    def kernel_vectors(self, v, gens):
+        v = _u32(v).reshape(-1)
+        gens = _u32(gens).reshape(-1)
+        b = self.block
+        ngen = gens.size // (b * b)
+        d = np.empty(self.cols * b, np.uint32)
+        prod = np.empty(self.rows * b, np.uint32)
+        _check(lib().pgw_kernel_vectors(self.h, _ptr(v), _ptr(gens), ngen, _ptr(d), _ptr(prod)))
+        return d, prod
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgw_operator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

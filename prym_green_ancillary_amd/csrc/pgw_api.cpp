@@ -1,0 +1,680 @@
+// pgw_api.cpp -- the C ABI of libpgw: contexts, operators, the block-apply
+// chain, the Gram projection, the sequence loop and kernel-vector recovery.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pgw_kernels.h"
+
+using namespace pgw;
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PGW_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return PGW_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return PGW_ERR_INVALID;
+    }
+}
+
+struct DeviceArena {
+    std::vector<void*> ptrs;
+    uint64_t bytes = 0;
+    template <typename T>
+    T* alloc(uint64_t count) {
+        if (count == 0) count = 1;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess)
+            throw Error(PGW_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                            " bytes failed: " + hipGetErrorString(e));
+        ptrs.push_back(p);
+        bytes += count * sizeof(T);
+        return (T*)p;
+    }
+    template <typename T>
+    T* upload(const T* host, uint64_t count) {
+        T* d = alloc<T>(count);
+        if (count) PGW_HIP(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+        return d;
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& v) { return upload(v.data(), (uint64_t)v.size()); }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+        bytes = 0;
+    }
+};
+
+PhiDevice upload_phi(DeviceArena& A, const PhiHost& h) {
+    PhiDevice d;
+    d.nI = h.nI; d.nB = h.nB; d.nA = h.nA; d.r = h.r;
+    d.rows = h.row_rank.size();
+    d.cols = h.col_rank.size();
+    d.row_subsets = h.row_subsets;
+    d.col_subsets = h.col_subsets;
+    d.row_width = h.row_width;
+    d.col_width = h.col_width;
+    d.mu = A.upload(h.mu);
+    d.wV = A.upload(h.wV);
+    d.wA0 = A.upload(h.wA0);
+    d.wA1 = A.upload(h.wA1);
+    d.a0_off = A.upload(h.a0_off);
+    d.a0_idx = A.upload(h.a0_idx);
+    d.a1_off = A.upload(h.a1_off);
+    d.a1_idx = A.upload(h.a1_idx);
+    d.row_off = A.upload(h.row_off);
+    d.row_rank = A.upload(h.row_rank);
+    d.row_beta = A.upload(h.row_beta);
+    d.col_off = A.upload(h.col_off);
+    d.col_rank = A.upload(h.col_rank);
+    d.col_a = A.upload(h.col_a);
+    d.row_inc = A.upload(h.row_inc_code);
+    d.row_inc_rank = A.upload(h.row_inc_rank);
+    d.col_inc = A.upload(h.col_inc_code);
+    d.col_inc_rank = A.upload(h.col_inc_rank);
+    return d;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+struct pgw_op {
+    pgw_ctx* ctx = nullptr;
+    uint32_t kind = 0, p = 0, b = 0, method = PGW_METHOD_PLAIN;
+    ModP m{};
+    uint64_t rows = 0, cols = 0, mid = 0;
+    DeviceArena arena;
+    uint32_t* d_dr = nullptr;
+    uint32_t* d_dc = nullptr;
+    uint32_t* d_negdc = nullptr;
+    pgw_rowmix_pair* d_pairs = nullptr;
+    uint32_t rounds = 0;
+    uint64_t pair_count = 0;
+    int64_t unpaired = -1;
+    uint32_t* d_scaled = nullptr;  // cols x b
+    uint32_t* d_mid = nullptr;     // mid x b
+    uint32_t* d_target = nullptr;  // rows x b
+    uint64_t* d_gram = nullptr;
+    // synthetic
+    uint32_t n = 0, k = 0, M = 0, Mp = 0;
+    uint64_t u_src = 0, u_tgt = 0;
+    uint32_t* d_tab[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t k4[4] = {0, 0, 0, 0};
+    MfmaTable mt[4];
+    uint32_t nl = 0;
+    int8_t* d_lp_src = nullptr;
+    int8_t* d_lp_mid = nullptr;
+    // cyclic eliminated
+    PhiDevice dm, d2;
+    uint32_t* d_zoff = nullptr;
+    uint32_t* d_zrank = nullptr;
+    uint16_t* d_zkappa = nullptr;
+    uint32_t* d_R = nullptr;
+    uint32_t* d_Bk = nullptr;
+    uint32_t n_v = 0, kdim = 0;
+    uint64_t x_cols = 0, z_count = 0;
+    uint32_t* d_t = nullptr;
+    uint32_t* d_y = nullptr;
+    uint32_t* d_h = nullptr;
+    uint32_t* d_tadj = nullptr;
+    // timing of the sweep kernels (pgw_bench_apply)
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> sweep_events;
+};
+
+namespace {
+
+DenseSweep sweep_desc(const pgw_op* op, int which) {
+    DenseSweep s;
+    s.n = op->n;
+    s.k = op->k;
+    s.fwd = which == 0 || which == 2;
+    s.u_out = s.fwd ? op->u_tgt : op->u_src;
+    s.v_in = s.fwd ? op->u_src : op->u_tgt;
+    s.r_out = s.fwd ? op->Mp : op->M;
+    s.k_in = s.fwd ? op->M : op->Mp;
+    return s;
+}
+
+void check_mfma_bound(const DenseSweep& s, uint32_t nl) {
+    // int32 accumulators: incidences * K_in * (pairs per shift <= NL) * 2^14 < 2^31
+    const uint64_t inc = s.fwd ? (s.n - s.k + 1) : s.k;
+    require(inc * s.k_in * nl * (1ULL << 14) < (1ULL << 31),
+            "int8-limb accumulation bound exceeded for this operator (incidences*K*limbs >= 2^17)",
+            PGW_ERR_UNSUPPORTED);
+}
+
+struct SweepTimer {
+    pgw_op* op;
+    hipStream_t st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SweepTimer(pgw_op* o, hipStream_t s) : op(o), st(s) {
+        if (op->profile) {
+            PGW_HIP(hipEventCreate(&e0));
+            PGW_HIP(hipEventCreate(&e1));
+            PGW_HIP(hipEventRecord(e0, st));
+        }
+    }
+    void done() {
+        if (op->profile) {
+            PGW_HIP(hipEventRecord(e1, st));
+            op->sweep_events.emplace_back(e0, e1);
+        }
+    }
+};
+
+// in: u32 block (plain) ; the MFMA path consumes the digit planes prepared by
+// the caller in op->d_lp_src / d_lp_mid
+void run_sweep(pgw_op* op, int which, const uint32_t* in_u32, const int8_t* in_lp, uint32_t* out,
+               const uint32_t* scale, hipStream_t st) {
+    const DenseSweep s = sweep_desc(op, which);
+    SweepTimer t(op, st);
+    if (op->method == PGW_METHOD_MFMA)
+        launch_dense_mfma(s, op->mt[which], in_lp, out, scale, op->b, op->nl, op->m, st);
+    else
+        launch_dense_plain(s, op->d_tab[which][0], op->d_tab[which][1], op->k4[which], in_u32, out,
+                           scale, op->b, op->m, st);
+    t.done();
+}
+
+// F * (Dc x) into op->d_target (rows x b)
+void forward_into_target(pgw_op* op, const uint32_t* x, hipStream_t st) {
+    const ModP& m = op->m;
+    if (op->kind == PGW_KIND_SYNTHETIC) {
+        if (op->method == PGW_METHOD_MFMA) {
+            launch_limb_split(x, op->d_dc, op->d_lp_src, op->u_src, op->M, op->b, op->nl, m, st);
+            run_sweep(op, 0, nullptr, op->d_lp_src, op->d_mid, nullptr, st);
+            launch_limb_split(op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, op->b, op->nl, m, st);
+            run_sweep(op, 1, nullptr, op->d_lp_mid, op->d_target, nullptr, st);
+        } else {
+            launch_scale_rows(x, op->d_dc, op->d_scaled, op->cols, op->b, m, st);
+            run_sweep(op, 0, op->d_scaled, nullptr, op->d_mid, nullptr, st);
+            run_sweep(op, 1, op->d_mid, nullptr, op->d_target, nullptr, st);
+        }
+    } else {
+        launch_scale_rows(x, op->d_dc, op->d_scaled, op->cols, op->b, m, st);
+        launch_cyclic_fwd(op->dm, op->d_scaled, op->d_t, nullptr, op->b, m, st);
+        launch_elim_y(op->dm, op->d2, op->d_zoff, op->d_zkappa, op->d_R, op->d_Bk, op->n_v, op->kdim,
+                      op->d_scaled + op->x_cols * op->b, op->d_t, op->d_y, op->b, m, st);
+        launch_cyclic_fwd(op->d2, op->d_y, op->d_target, nullptr, op->b, m, st);
+    }
+}
+
+// out = Dc F^T (op->d_target)
+void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
+    const ModP& m = op->m;
+    if (op->kind == PGW_KIND_SYNTHETIC) {
+        if (op->method == PGW_METHOD_MFMA) {
+            launch_limb_split(op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, op->b, op->nl, m, st);
+            run_sweep(op, 2, nullptr, op->d_lp_src, op->d_mid, nullptr, st);
+            launch_limb_split(op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, op->b, op->nl, m, st);
+            run_sweep(op, 3, nullptr, op->d_lp_mid, out, op->d_dc, st);
+        } else {
+            run_sweep(op, 2, op->d_target, nullptr, op->d_mid, nullptr, st);
+            run_sweep(op, 3, op->d_mid, nullptr, out, op->d_dc, st);
+        }
+    } else {
+        launch_cyclic_trn(op->d2, op->d_target, op->d_h, nullptr, op->b, m, st);
+        launch_elim_rt_h(op->dm, op->d2, op->d_R, op->n_v, op->d_h, op->d_tadj, op->b, m, st);
+        // -x_adj scaled by Dc: fold the sign into the scale (p - Dc)
+        launch_cyclic_trn(op->dm, op->d_tadj, out, op->d_negdc, op->b, m, st);
+        launch_elim_bt_h(op->d2, op->z_count, op->d_zrank, op->d_zkappa, op->d_Bk, op->kdim, op->d_h,
+                         out + op->x_cols * op->b, op->d_dc + op->x_cols, op->b, m, st);
+    }
+}
+
+void apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st) {
+    forward_into_target(op, in, st);
+    launch_target_precond(op->d_target, op->d_pairs, op->pair_count, op->rounds, op->d_dr,
+                          op->unpaired, op->rows, op->b, op->m, st);
+    backward_from_target(op, out, st);
+}
+
+hipStream_t pick_stream(pgw_op* op, void* stream) {
+    return stream ? (hipStream_t)stream : op->ctx->stream;
+}
+
+void validate_desc_common(const pgw_operator_desc* d) {
+    require(d != nullptr, "null operator description");
+    require(d->prime > 2 && d->prime < (1u << 31), "prime must satisfy 2 < p < 2^31");
+    for (uint32_t q = 2; (uint64_t)q * q <= d->prime; ++q)
+        require(d->prime % q != 0, "modulus " + std::to_string(d->prime) + " is not prime");
+    require(d->block >= 1 && d->block <= 16, "block width must be in 1..16");
+}
+
+void synthetic_dims(const pgw_operator_desc* d, uint64_t& rows, uint64_t& cols, uint64_t& mid) {
+    require(d->n >= 2 && d->n < 64, "synthetic operator needs 2 <= n < 64");
+    require(d->k >= 1 && d->k <= d->n, "synthetic operator needs 1 <= k <= n");
+    require(d->m_src >= 1 && d->m_tgt >= 1, "coefficient dimensions must be positive");
+    const uint64_t us = host_binom(d->n, d->k), ut = host_binom(d->n, d->k - 1);
+    require(us < (1ULL << 31) && ut < (1ULL << 31), "subset count too large");
+    cols = us * d->m_src;
+    rows = cols;
+    mid = ut * d->m_tgt;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int pgw_ctx_create(int device, pgw_ctx** out) {
+    return guarded([&] {
+        require(out != nullptr, "null output pointer");
+        int count = 0;
+        PGW_HIP(hipGetDeviceCount(&count));
+        require(device >= 0 && device < count, "device index out of range");
+        PGW_HIP(hipSetDevice(device));
+        auto* c = new pgw_ctx;
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw Error(PGW_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+        }
+        upload_binomials(device);
+        *out = c;
+    });
+}
+
+int pgw_ctx_destroy(pgw_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        PGW_HIP(hipSetDevice(ctx->device));
+        PGW_HIP(hipStreamSynchronize(ctx->stream));
+        PGW_HIP(hipStreamDestroy(ctx->stream));
+        delete ctx;
+    });
+}
+
+int pgw_ctx_synchronize(pgw_ctx* ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        PGW_HIP(hipSetDevice(ctx->device));
+        PGW_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pgw_ctx_device_info(pgw_ctx* ctx, char* name, size_t name_len, uint64_t* total_bytes) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        hipDeviceProp_t prop;
+        PGW_HIP(hipGetDeviceProperties(&prop, ctx->device));
+        if (name && name_len) {
+            std::strncpy(name, prop.name, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (total_bytes) *total_bytes = prop.totalGlobalMem;
+    });
+}
+
+int pgw_operator_dims(const pgw_operator_desc* desc, uint64_t* rows, uint64_t* cols) {
+    return guarded([&] {
+        validate_desc_common(desc);
+        uint64_t r = 0, c = 0, mid = 0;
+        if (desc->kind == PGW_KIND_SYNTHETIC) {
+            synthetic_dims(desc, r, c, mid);
+        } else if (desc->kind == PGW_KIND_CYCLIC_ELIMINATED) {
+            CyclicLayout L;
+            build_cyclic_layout(L, *desc);
+            r = L.rows;
+            c = L.cols;
+        } else {
+            throw Error(PGW_ERR_INVALID, "unknown operator kind");
+        }
+        if (rows) *rows = r;
+        if (cols) *cols = c;
+    });
+}
+
+int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
+                        const uint32_t* h_row_precond, const uint32_t* h_col_precond,
+                        uint32_t rowmix_rounds, const pgw_rowmix_pair* h_pairs, pgw_op** out) {
+    pgw_op* op = nullptr;
+    int rc = guarded([&] {
+        require(ctx != nullptr && out != nullptr, "null context or output pointer");
+        validate_desc_common(desc);
+        require(h_row_precond && h_col_precond, "preconditioners are required");
+        PGW_HIP(hipSetDevice(ctx->device));
+        op = new pgw_op;
+        op->ctx = ctx;
+        op->kind = desc->kind;
+        op->p = desc->prime;
+        op->b = desc->block;
+        op->m = make_modp(op->p);
+        DeviceArena& A = op->arena;
+        const uint32_t b = op->b;
+        if (desc->kind == PGW_KIND_SYNTHETIC) {
+            require(desc->table_d && desc->table_e, "synthetic operator needs both tables");
+            synthetic_dims(desc, op->rows, op->cols, op->mid);
+            op->n = desc->n;
+            op->k = desc->k;
+            op->M = desc->m_src;
+            op->Mp = desc->m_tgt;
+            op->u_src = host_binom(op->n, op->k);
+            op->u_tgt = host_binom(op->n, op->k - 1);
+            const uint64_t tsize = (uint64_t)op->n * op->Mp * op->M;
+            for (uint64_t q = 0; q < tsize; ++q)
+                require(desc->table_d[q] < op->p && desc->table_e[q] < op->p,
+                        "table entries must be reduced modulo p");
+            uint32_t method = desc->method;
+            const bool mfma_ok = mfma_block_supported(b) && op->n <= 32 && op->k >= 1;
+            if (method == PGW_METHOD_AUTO) method = mfma_ok ? PGW_METHOD_MFMA : PGW_METHOD_PLAIN;
+            require(method == PGW_METHOD_PLAIN || mfma_ok,
+                    "MFMA path needs b in {4, 8, 16} and n <= 32", PGW_ERR_UNSUPPORTED);
+            op->method = method;
+            // sweep tables: 0 = D (T, forward), 1 = E (E^T, transpose), 2 = E^T (E, forward),
+            // 3 = D^T (T^T, transpose)
+            const uint32_t* src[4] = {desc->table_d, desc->table_e, desc->table_e, desc->table_d};
+            if (method == PGW_METHOD_MFMA) {
+                op->nl = mfma_limbs(op->p);
+                for (int w = 0; w < 4; ++w) {
+                    DenseSweep s = sweep_desc(op, w);
+                    check_mfma_bound(s, op->nl);
+                    build_mfma_table(op->mt[w], op->n, s.r_out, s.k_in, op->p, src[w], !s.fwd);
+                    A.ptrs.push_back(op->mt[w].d);
+                    A.bytes += (uint64_t)op->n * op->nl * op->mt[w].rows_pad * op->mt[w].k_pad;
+                }
+                op->d_lp_src = A.alloc<int8_t>(limb_plane_bytes(op->u_src, op->M, b, op->nl));
+                op->d_lp_mid = A.alloc<int8_t>(limb_plane_bytes(op->u_tgt, op->Mp, b, op->nl));
+            } else {
+                for (int w = 0; w < 4; ++w) {
+                    DenseSweep s = sweep_desc(op, w);
+                    const uint32_t k4 = (s.k_in + 3) / 4 * 4;
+                    op->k4[w] = k4;
+                    std::vector<uint32_t> pos((uint64_t)op->n * s.r_out * k4, 0), neg(pos.size(), 0);
+                    for (uint32_t i = 0; i < op->n; ++i)
+                        for (uint32_t r = 0; r < s.r_out; ++r)
+                            for (uint32_t kk = 0; kk < s.k_in; ++kk) {
+                                // table [i][beta][a]; forward rows are beta, transpose rows a
+                                uint32_t v = s.fwd ? src[w][((uint64_t)i * op->Mp + r) * op->M + kk]
+                                                   : src[w][((uint64_t)i * op->Mp + kk) * op->M + r];
+                                pos[((uint64_t)i * s.r_out + r) * k4 + kk] = v;
+                                neg[((uint64_t)i * s.r_out + r) * k4 + kk] = v ? op->p - v : 0;
+                            }
+                    op->d_tab[w][0] = A.upload(pos);
+                    op->d_tab[w][1] = A.upload(neg);
+                }
+                op->d_scaled = A.alloc<uint32_t>(op->cols * b);
+            }
+            op->d_mid = A.alloc<uint32_t>(op->mid * b);
+        } else if (desc->kind == PGW_KIND_CYCLIC_ELIMINATED) {
+            require(desc->method != PGW_METHOD_MFMA,
+                    "the cyclic operator's character blocks run on the plain path", PGW_ERR_UNSUPPORTED);
+            op->method = PGW_METHOD_PLAIN;
+            CyclicLayout L;
+            build_cyclic_layout(L, *desc);
+            op->rows = L.rows;
+            op->cols = L.cols;
+            op->mid = L.d2.col_rank.size();
+            op->dm = upload_phi(A, L.dm);
+            op->d2 = upload_phi(A, L.d2);
+            op->d_zoff = A.upload(L.z_off);
+            op->d_zrank = A.upload(L.z_rank);
+            op->d_zkappa = A.upload(L.z_kappa);
+            const uint32_t g = desc->genus, nv = g - 3;
+            op->n_v = nv;
+            op->kdim = desc->kernel_dim;
+            for (uint64_t q = 0; q < (uint64_t)nv * nv * g; ++q)
+                require(desc->mu[q] < op->p, "mu.values entry is not reduced modulo the instance modulus");
+            op->d_R = A.upload(desc->right_inverse, (uint64_t)g * nv);
+            op->d_Bk = A.upload(desc->kernel_basis, (uint64_t)g * desc->kernel_dim);
+            op->x_cols = L.x_cols;
+            op->z_count = L.z_count;
+            op->d_scaled = A.alloc<uint32_t>(op->cols * b);
+            op->d_t = A.alloc<uint32_t>(L.dm.row_rank.size() * b);
+            op->d_y = A.alloc<uint32_t>(op->mid * b);
+            op->d_h = A.alloc<uint32_t>(op->mid * b);
+            op->d_tadj = A.alloc<uint32_t>(L.dm.row_rank.size() * b);
+        } else {
+            throw Error(PGW_ERR_INVALID, "unknown operator kind");
+        }
+        // preconditioners and the row-mixing plan
+        for (uint64_t q = 0; q < op->rows; ++q)
+            require(h_row_precond[q] >= 1 && h_row_precond[q] < op->p, "row preconditioner entries must be nonzero residues");
+        for (uint64_t q = 0; q < op->cols; ++q)
+            require(h_col_precond[q] >= 1 && h_col_precond[q] < op->p, "column preconditioner entries must be nonzero residues");
+        op->d_dr = A.upload(h_row_precond, op->rows);
+        op->d_dc = A.upload(h_col_precond, op->cols);
+        std::vector<uint32_t> negdc(op->cols);
+        for (uint64_t q = 0; q < op->cols; ++q) negdc[q] = op->p - h_col_precond[q];
+        op->d_negdc = A.upload(negdc);
+        op->rounds = rowmix_rounds;
+        op->pair_count = op->rows / 2;
+        if (rowmix_rounds > 0 && op->pair_count > 0) {
+            require(h_pairs != nullptr, "row-mix plan is required when rounds > 0");
+            const uint64_t total = (uint64_t)rowmix_rounds * op->pair_count;
+            std::vector<uint8_t> seen(op->rows);
+            for (uint32_t r = 0; r < rowmix_rounds; ++r) {
+                std::fill(seen.begin(), seen.end(), 0);
+                for (uint64_t q = 0; q < op->pair_count; ++q) {
+                    const pgw_rowmix_pair& pr = h_pairs[(uint64_t)r * op->pair_count + q];
+                    require(pr.first < op->rows && pr.second < op->rows && pr.first != pr.second,
+                            "row-mix pair outside the target rows");
+                    require(!seen[pr.first] && !seen[pr.second], "row-mix pairs of a round must be disjoint");
+                    require(pr.a >= 1 && pr.a < op->p && pr.b >= 1 && pr.b < op->p,
+                            "row-mix coefficients must be nonzero residues");
+                    seen[pr.first] = seen[pr.second] = 1;
+                }
+            }
+            op->unpaired = -1;
+            if (op->rows % 2 == 1)
+                for (uint64_t q = 0; q < op->rows; ++q)
+                    if (!seen[q]) op->unpaired = (int64_t)q;  // left out of the last round
+            op->d_pairs = A.upload(h_pairs, total);
+        } else {
+            op->rounds = 0;
+        }
+        op->d_target = A.alloc<uint32_t>(op->rows * b);
+        op->d_gram = A.alloc<uint64_t>(gram_scratch_bytes(b) / sizeof(uint64_t));
+        *out = op;
+    });
+    if (rc != PGW_OK && op) {
+        op->arena.release();
+        delete op;
+    }
+    return rc;
+}
+
+int pgw_operator_destroy(pgw_op* op) {
+    return guarded([&] {
+        if (!op) return;
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        PGW_HIP(hipStreamSynchronize(op->ctx->stream));
+        op->arena.release();
+        for (auto& e : op->sweep_events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        delete op;
+    });
+}
+
+int pgw_operator_info(const pgw_op* op, pgw_op_info* info) {
+    return guarded([&] {
+        require(op && info, "null operator or info");
+        info->rows = op->rows;
+        info->cols = op->cols;
+        info->mid_rows = op->mid;
+        info->device_bytes = op->arena.bytes;
+        info->method = op->method;
+        info->block = op->b;
+    });
+}
+
+int pgw_apply_square(pgw_op* op, const uint32_t* d_in, uint32_t* d_out, void* stream) {
+    return guarded([&] {
+        require(op && d_in && d_out, "null operator or block");
+        require(d_in != d_out, "pgw_apply_square needs distinct input and output blocks");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        apply_square_impl(op, d_in, d_out, pick_stream(op, stream));
+    });
+}
+
+int pgw_apply_forward(pgw_op* op, const uint32_t* d_in, uint32_t* d_out, void* stream) {
+    return guarded([&] {
+        require(op && d_in && d_out, "null operator or block");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        hipStream_t st = pick_stream(op, stream);
+        forward_into_target(op, d_in, st);
+        PGW_HIP(hipMemcpyAsync(d_out, op->d_target, op->rows * op->b * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, st));
+    });
+}
+
+int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_t* d_out,
+                   void* stream) {
+    return guarded([&] {
+        require(op && d_x && d_y && d_out, "null operator or block");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        launch_gram(d_x, d_y, op->cols, op->b, 0, d_out, nullptr, op->d_gram, op->m,
+                    pick_stream(op, stream));
+    });
+}
+
+int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t* h_seq,
+                     uint32_t* h_curv, uint64_t* out_len, pgw_progress_fn progress, void* user) {
+    uint32_t* d_B = nullptr;
+    uint32_t* d_D = nullptr;
+    uint32_t* d_seq = nullptr;
+    int rc = guarded([&] {
+        require(op && h_v && h_seq && h_curv, "null operator or buffer");
+        require(max_len >= 1, "sequence length must be positive");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        hipStream_t st = op->ctx->stream;
+        const uint64_t N = op->cols, b = op->b, ut = b * (b + 1) / 2;
+        const uint64_t rounds = (max_len - 1 + 3) / 4;
+        const uint64_t L = 1 + 4 * rounds;
+        for (uint64_t q = 0; q < N * b; ++q) require(h_v[q] < op->p, "V entries must be reduced modulo p");
+        PGW_HIP(hipMalloc(&d_B, N * b * sizeof(uint32_t)));
+        PGW_HIP(hipMalloc(&d_D, N * b * sizeof(uint32_t)));
+        PGW_HIP(hipMalloc(&d_seq, L * ut * sizeof(uint32_t)));
+        PGW_HIP(hipMemcpyAsync(d_B, h_v, N * b * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        uint64_t pos = 0;
+        launch_gram(d_B, d_B, N, op->b, 0, d_seq + ut * pos, nullptr, op->d_gram, op->m, st);
+        ++pos;
+        for (uint64_t r = 0; r < rounds; ++r) {
+            apply_square_impl(op, d_B, d_D, st);
+            launch_gram(d_B, d_D, N, op->b, 1, d_seq + ut * pos, d_seq + ut * (pos + 1), op->d_gram, op->m, st);
+            pos += 2;
+            apply_square_impl(op, d_D, d_B, st);
+            launch_gram(d_B, d_D, N, op->b, 2, d_seq + ut * pos, d_seq + ut * (pos + 1), op->d_gram, op->m, st);
+            pos += 2;
+            if (progress && ((r + 1) % 64 == 0 || r + 1 == rounds)) {
+                PGW_HIP(hipStreamSynchronize(st));  // also bounds the launch queue
+                progress(pos, L, user);
+            }
+        }
+        PGW_HIP(hipMemcpyAsync(h_seq, d_seq, L * ut * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        PGW_HIP(hipMemcpyAsync(h_curv, d_B, N * b * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        PGW_HIP(hipStreamSynchronize(st));
+        if (out_len) *out_len = L;
+    });
+    if (d_B) (void)hipFree(d_B);
+    if (d_D) (void)hipFree(d_D);
+    if (d_seq) (void)hipFree(d_seq);
+    return rc;
+}
+
+int pgw_kernel_vectors(pgw_op* op, const uint32_t* h_v, const uint32_t* h_gens, uint64_t ngen,
+                       uint32_t* h_d, uint32_t* h_product) {
+    uint32_t* d_B = nullptr;
+    uint32_t* d_N = nullptr;
+    uint32_t* d_D = nullptr;
+    uint32_t* d_G = nullptr;
+    int rc = guarded([&] {
+        require(op && h_v && h_gens && h_d && h_product, "null operator or buffer");
+        require(ngen >= 1, "generator file does not contain any generator matrices");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        hipStream_t st = op->ctx->stream;
+        const uint64_t N = op->cols, b = op->b;
+        PGW_HIP(hipMalloc(&d_B, N * b * sizeof(uint32_t)));
+        PGW_HIP(hipMalloc(&d_N, N * b * sizeof(uint32_t)));
+        PGW_HIP(hipMalloc(&d_D, N * b * sizeof(uint32_t)));
+        PGW_HIP(hipMalloc(&d_G, ngen * b * b * sizeof(uint32_t)));
+        PGW_HIP(hipMemcpyAsync(d_B, h_v, N * b * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        PGW_HIP(hipMemcpyAsync(d_G, h_gens, ngen * b * b * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        PGW_HIP(hipMemsetAsync(d_D, 0, N * b * sizeof(uint32_t), st));
+        for (uint64_t kk = 0; kk < ngen; ++kk) {
+            launch_accumulate_generator(d_B, d_G + kk * b * b, d_D, N, op->b, op->m, st);
+            if (kk + 1 < ngen) {
+                apply_square_impl(op, d_B, d_N, st);
+                std::swap(d_B, d_N);
+            }
+            if ((kk + 1) % 256 == 0) PGW_HIP(hipStreamSynchronize(st));
+        }
+        forward_into_target(op, d_D, st);
+        PGW_HIP(hipMemcpyAsync(h_d, d_D, N * b * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        PGW_HIP(hipMemcpyAsync(h_product, op->d_target, op->rows * b * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, st));
+        PGW_HIP(hipStreamSynchronize(st));
+    });
+    for (uint32_t* p : {d_B, d_N, d_D, d_G})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, double* total_ms,
+                    double* sweep_avg_ms, uint32_t* sweeps_per_apply) {
+    return guarded([&] {
+        require(op && d_a && d_b && iters >= 1, "null operator/block or zero iterations");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        hipStream_t st = op->ctx->stream;
+        for (auto& e : op->sweep_events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        op->sweep_events.clear();
+        hipEvent_t t0, t1;
+        PGW_HIP(hipEventCreate(&t0));
+        PGW_HIP(hipEventCreate(&t1));
+        PGW_HIP(hipStreamSynchronize(st));
+        // timed region: iters block-applies, no per-kernel events
+        PGW_HIP(hipEventRecord(t0, st));
+        for (uint32_t it = 0; it < iters; ++it)
+            apply_square_impl(op, (it % 2) ? d_b : d_a, (it % 2) ? d_a : d_b, st);
+        PGW_HIP(hipEventRecord(t1, st));
+        PGW_HIP(hipEventSynchronize(t1));
+        float ms = 0;
+        PGW_HIP(hipEventElapsedTime(&ms, t0, t1));
+        if (total_ms) *total_ms = ms;
+        // per-launch sweep durations: one more apply with events around each sweep
+        op->profile = op->kind == PGW_KIND_SYNTHETIC;
+        apply_square_impl(op, (iters % 2) ? d_b : d_a, (iters % 2) ? d_a : d_b, st);
+        op->profile = false;
+        PGW_HIP(hipStreamSynchronize(st));
+        double sum = 0;
+        for (auto& e : op->sweep_events) {
+            float s = 0;
+            PGW_HIP(hipEventElapsedTime(&s, e.first, e.second));
+            sum += s;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        const size_t count = op->sweep_events.size();
+        op->sweep_events.clear();
+        if (sweep_avg_ms) *sweep_avg_ms = count ? sum / (double)count : 0.0;
+        if (sweeps_per_apply) *sweeps_per_apply = (uint32_t)count;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+    });
+}
+
+}  // extern "C"

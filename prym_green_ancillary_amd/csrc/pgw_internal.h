@@ -1,0 +1,216 @@
+// pgw_internal.h -- shared declarations of libpgw (host structs, device helpers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pgw.h"
+
+namespace pgw {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define PGW_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t pgw_e_ = (call);                                                     \
+        if (pgw_e_ != hipSuccess)                                                       \
+            throw ::pgw::Error(PGW_ERR_HIP, std::string(#call " failed: ") +           \
+                                                hipGetErrorString(pgw_e_));             \
+    } while (0)
+
+#define PGW_CHECK_LAUNCH(name)                                                          \
+    do {                                                                                \
+        hipError_t pgw_e_ = hipGetLastError();                                          \
+        if (pgw_e_ != hipSuccess)                                                       \
+            throw ::pgw::Error(PGW_ERR_HIP, std::string(name " launch failed: ") +     \
+                                                hipGetErrorString(pgw_e_));             \
+    } while (0)
+
+inline void require(bool cond, const std::string& msg, int status = PGW_ERR_INVALID) {
+    if (!cond) throw Error(status, msg);
+}
+
+// ---------------------------------------------------------------------------
+// modular arithmetic (p prime, 2 < p < 2^31)
+// ---------------------------------------------------------------------------
+
+struct ModP {
+    uint32_t p;
+    uint32_t r32;     // 2^32 mod p
+    uint64_t barrett; // floor(2^64 / p)
+};
+
+inline ModP make_modp(uint32_t p) {
+    ModP m;
+    m.p = p;
+    m.r32 = (uint32_t)((1ULL << 32) % p);
+    m.barrett = (uint64_t)(((unsigned __int128)1 << 64) / p);
+    return m;
+}
+
+// x mod p for any 64-bit x: Barrett with m = floor(2^64/p) leaves r < 3p.
+__host__ __device__ inline uint32_t mod_u64(uint64_t x, const ModP& m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t q = __umul64hi(x, m.barrett);
+#else
+    uint64_t q = (uint64_t)(((unsigned __int128)x * m.barrett) >> 64);
+#endif
+    uint64_t r = x - q * (uint64_t)m.p;
+    if (r >= m.p) r -= m.p;
+    if (r >= m.p) r -= m.p;
+    return (uint32_t)r;
+}
+
+__host__ __device__ inline uint32_t mulmod(uint32_t a, uint32_t b, const ModP& m) {
+    return mod_u64((uint64_t)a * b, m);
+}
+
+__host__ __device__ inline uint32_t addmod(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t s = a + b;  // a, b < p < 2^31: no wrap
+    return s >= p ? s - p : s;
+}
+
+// (hi * 2^32 + lo) mod p for the split accumulators of the plain kernels.
+__host__ __device__ inline uint32_t mod_split(uint64_t hi, uint64_t lo, const ModP& m) {
+    uint64_t h = mod_u64(hi, m);
+    return mod_u64(h * m.r32 + mod_u64(lo, m), m);
+}
+
+// ---------------------------------------------------------------------------
+// exterior-algebra basis: lexicographic k-subsets of [0,n) (n < 64)
+// ---------------------------------------------------------------------------
+
+// Uploads the binomial table C(a,b), a,b < 64, to the device constant memory
+// of the kernel module (once per device).
+void upload_binomials(int device);
+uint64_t host_binom(uint32_t n, uint32_t k);
+
+template <typename Binom>
+__host__ __device__ inline uint64_t unrank_mask(uint64_t rank, int n, int k, const Binom& C) {
+    // greedy lexicographic unrank (same order as the reference's unrank_subset)
+    uint64_t mask = 0;
+    int start = 0;
+    for (int pos = 0; pos < k; ++pos) {
+        int remaining = k - pos - 1;
+        for (int value = start; value < n; ++value) {
+            uint64_t count = C(n - value - 1, remaining);
+            if (rank < count) {
+                mask |= 1ULL << value;
+                start = value + 1;
+                break;
+            }
+            rank -= count;
+        }
+    }
+    return mask;
+}
+
+// Lexicographic rank of a k-subset given as a bit mask:
+// rank = sum_pos [ C(n - s_{pos-1} - 1, k - pos) - C(n - s_pos, k - pos) ]
+// (hockey-stick sum of the greedy unrank counts, s_{-1} = -1).
+template <typename Binom>
+__host__ __device__ inline uint64_t rank_mask(uint64_t mask, int n, int k, const Binom& C) {
+    uint64_t r = 0;
+    int prev = -1, pos = 0;
+    while (mask) {
+        int s = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        r += C(n - prev - 1, k - pos) - C(n - s, k - pos);
+        prev = s;
+        ++pos;
+    }
+    return r;
+}
+
+struct HostBinom {
+    uint64_t operator()(int a, int b) const {
+        return (a < 0 || b < 0 || b > a) ? 0ULL : host_binom((uint32_t)a, (uint32_t)b);
+    }
+};
+
+}  // namespace pgw
+
+// ---------------------------------------------------------------------------
+// host-visible objects
+// ---------------------------------------------------------------------------
+
+struct pgw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace pgw {
+
+// Character-restricted differential of the cyclic operator, as device arrays
+// (the reference's CudaCyclicPhiOperator, cuda/cyclic_cuda_helpers.h:268-296).
+struct PhiDevice {
+    uint32_t nI = 0, nB = 0, nA = 0, r = 1;
+    uint64_t rows = 0, cols = 0, row_subsets = 0, col_subsets = 0;
+    uint32_t row_width = 0, col_width = 0;
+    uint32_t* mu = nullptr;          // [nI][nB][nA]
+    uint16_t* wV = nullptr;
+    uint16_t* wA0 = nullptr;
+    uint16_t* wA1 = nullptr;
+    uint32_t* a0_off = nullptr;
+    uint16_t* a0_idx = nullptr;
+    uint32_t* a1_off = nullptr;
+    uint16_t* a1_idx = nullptr;
+    uint32_t* row_off = nullptr;     // [row_subsets+1]
+    uint32_t* row_rank = nullptr;    // [rows]
+    uint16_t* row_beta = nullptr;    // [rows]
+    uint32_t* col_off = nullptr;     // [col_subsets+1]
+    uint32_t* col_rank = nullptr;    // [cols]
+    uint16_t* col_a = nullptr;       // [cols]
+    uint32_t* row_inc = nullptr;     // [row_subsets*row_width] packed (i<<25 | neg<<24) pairs: see kernels
+    uint32_t* row_inc_rank = nullptr;
+    uint32_t* col_inc = nullptr;
+    uint32_t* col_inc_rank = nullptr;
+};
+
+// Host-side layout of one differential (built like build_common_incidences_generic
+// + build_sector_layout_generic, cuda/cuprym_cyclic.cu:631-790).
+struct PhiHost {
+    uint32_t nI = 0, nB = 0, nA = 0, r = 1;
+    std::vector<uint32_t> mu;
+    std::vector<uint16_t> wV, wA0, wA1;
+    std::vector<uint32_t> a0_off, a1_off;
+    std::vector<uint16_t> a0_idx, a1_idx;
+    uint64_t row_subsets = 0, col_subsets = 0;
+    uint32_t row_width = 0, col_width = 0;
+    std::vector<uint16_t> row_subset_w, col_subset_w;
+    std::vector<uint32_t> row_inc_code, row_inc_rank;  // code = i | neg<<31
+    std::vector<uint32_t> col_inc_code, col_inc_rank;
+    std::vector<uint32_t> row_off, row_rank, col_off, col_rank;
+    std::vector<uint16_t> row_beta, col_a;
+};
+
+void build_char_map(const std::vector<uint16_t>& w, uint32_t r, std::vector<uint32_t>& off,
+                    std::vector<uint16_t>& idx);
+void build_phi_host(PhiHost& h, uint32_t nI, uint32_t nB, uint32_t nA, uint32_t r,
+                    const uint32_t* mu, const uint16_t* wV, const uint16_t* wA0,
+                    const uint16_t* wA1, const std::vector<uint32_t>& original, uint32_t degree,
+                    uint32_t sector);
+
+// Cyclic eliminated layout sizes and z layout.
+struct CyclicLayout {
+    PhiHost dm, d2;
+    std::vector<uint32_t> z_off, z_rank;
+    std::vector<uint16_t> z_kappa;
+    uint64_t x_cols = 0, z_count = 0, rows = 0, cols = 0;
+};
+void build_cyclic_layout(CyclicLayout& L, const pgw_operator_desc& d);
+
+}  // namespace pgw

@@ -1,0 +1,80 @@
+// pgw_kernels.h -- launchers of the gfx950 kernels (pgw_kernels.hip).
+#pragma once
+
+#include "pgw_internal.h"
+
+namespace pgw {
+
+// Table-stationary digit planes of a synthetic multiplication table for the
+// MFMA sweep: tabl[i][limb][R_pad][K_pad] int8 (balanced base-256 digits).
+struct MfmaTable {
+    int8_t* d = nullptr;
+    uint32_t rows = 0, k_in = 0, rows_pad = 0, k_pad = 0;
+};
+
+// Launch description of one dense-table (r = 1) Koszul sweep.
+struct DenseSweep {
+    bool fwd = true;          // forward gather (out: (k-1)-subsets) or transpose (out: k-subsets)
+    uint32_t n = 0, k = 0;    // exterior algebra Lambda^k of an n-dim space
+    uint64_t u_out = 0;       // output subsets
+    uint64_t v_in = 0;        // input subsets
+    uint32_t r_out = 0;       // output rows per subset panel
+    uint32_t k_in = 0;        // input rows per subset panel
+};
+
+// plain u64 path ------------------------------------------------------------
+void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t* tab_neg,
+                        uint32_t k4, const uint32_t* in, uint32_t* out, const uint32_t* scale,
+                        uint32_t b, const ModP& m, hipStream_t st);
+void launch_cyclic_fwd(const PhiDevice& d, const uint32_t* in, uint32_t* out,
+                       const uint32_t* scale, uint32_t b, const ModP& m, hipStream_t st);
+void launch_cyclic_trn(const PhiDevice& d, const uint32_t* in, uint32_t* out,
+                       const uint32_t* scale, uint32_t b, const ModP& m, hipStream_t st);
+void launch_elim_y(const PhiDevice& dm, const PhiDevice& d2, const uint32_t* z_off,
+                   const uint16_t* z_kappa, const uint32_t* right_inv, const uint32_t* kbasis,
+                   uint32_t n_v, uint32_t kdim, const uint32_t* z, const uint32_t* t, uint32_t* y,
+                   uint32_t b, const ModP& m, hipStream_t st);
+void launch_elim_rt_h(const PhiDevice& dm, const PhiDevice& d2, const uint32_t* right_inv,
+                      uint32_t n_v, const uint32_t* h, uint32_t* t_adj, uint32_t b,
+                      const ModP& m, hipStream_t st);
+void launch_elim_bt_h(const PhiDevice& d2, uint64_t z_count, const uint32_t* z_rank,
+                      const uint16_t* z_kappa, const uint32_t* kbasis, uint32_t kdim,
+                      const uint32_t* h, uint32_t* z_adj, const uint32_t* scale, uint32_t b,
+                      const ModP& m, hipStream_t st);
+
+// elementwise -----------------------------------------------------------------
+void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uint64_t rows,
+                       uint32_t b, const ModP& m, hipStream_t st);
+// P^T Dr P on a target block in place (P = `rounds` rounds of paired 2x2 maps).
+// `unpaired_row` is the row missing from the last round's pairs (odd row count) or -1.
+void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pair_count,
+                           uint32_t rounds, const uint32_t* dr, int64_t unpaired_row,
+                           uint64_t rows, uint32_t b, const ModP& m, hipStream_t st);
+
+// Gram projection -----------------------------------------------------------
+// mode 0: out0 = X^T Y; mode 1: out0 = X^T Y, out1 = Y^T Y; mode 2: out0 = X^T Y, out1 = X^T X.
+size_t gram_scratch_bytes(uint32_t b);
+void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
+                 uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
+                 hipStream_t st);
+
+// kernel-vector generator slice: D[r][c] += sum_i B[r][i] G[c*b+i]
+void launch_accumulate_generator(const uint32_t* B, const uint32_t* G, uint32_t* D, uint64_t rows,
+                                 uint32_t b, const ModP& m, hipStream_t st);
+
+// int8-limb MFMA path -----------------------------------------------------------
+uint32_t mfma_limbs(uint32_t p);  // 2 for p <= 65536, else 4
+bool mfma_block_supported(uint32_t b);
+// host: digits of a table for the MFMA sweep (rows x k_in view given by accessor)
+void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, uint32_t p,
+                      const uint32_t* src, bool transpose_src);
+// per-sweep digit-plane layout
+uint64_t limb_plane_bytes(uint64_t v, uint32_t k_in, uint32_t b, uint32_t nl);
+uint32_t limb_stride(uint32_t k_in);
+void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
+                       uint32_t k_in, uint32_t b, uint32_t nl, const ModP& m, hipStream_t st);
+void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
+                       const uint32_t* scale, uint32_t b, uint32_t nl, const ModP& m,
+                       hipStream_t st);
+
+}  // namespace pgw
