@@ -1,0 +1,265 @@
+// wdm.cpp -- WDM text writer/reader with an optional zstd frame.
+
+#include "wdm.h"
+
+#include <charconv>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+
+// libzstd's stable streaming API (zstd.h is not part of this image, the
+// shared library libzstd.so.1 is): opaque stream types and buffer structs.
+extern "C" {
+struct ZSTD_CCtx_s;
+struct ZSTD_DCtx_s;
+typedef struct ZSTD_CCtx_s ZSTD_CStream;
+typedef struct ZSTD_DCtx_s ZSTD_DStream;
+typedef struct {
+    const void* src;
+    size_t size;
+    size_t pos;
+} ZSTD_inBuffer;
+typedef struct {
+    void* dst;
+    size_t size;
+    size_t pos;
+} ZSTD_outBuffer;
+ZSTD_CStream* ZSTD_createCStream(void);
+size_t ZSTD_freeCStream(ZSTD_CStream*);
+size_t ZSTD_initCStream(ZSTD_CStream*, int level);
+size_t ZSTD_compressStream(ZSTD_CStream*, ZSTD_outBuffer*, ZSTD_inBuffer*);
+size_t ZSTD_endStream(ZSTD_CStream*, ZSTD_outBuffer*);
+size_t ZSTD_CStreamOutSize(void);
+ZSTD_DStream* ZSTD_createDStream(void);
+size_t ZSTD_freeDStream(ZSTD_DStream*);
+size_t ZSTD_initDStream(ZSTD_DStream*);
+size_t ZSTD_decompressStream(ZSTD_DStream*, ZSTD_outBuffer*, ZSTD_inBuffer*);
+size_t ZSTD_DStreamOutSize(void);
+unsigned ZSTD_isError(size_t);
+const char* ZSTD_getErrorName(size_t);
+}
+
+namespace pgwio {
+
+static bool ends_with(const std::string& s, const std::string& suf) {
+    return s.size() >= suf.size() && s.compare(s.size() - suf.size(), suf.size(), suf) == 0;
+}
+
+bool path_requests_compression(const std::string& path) {
+    return ends_with(path, ".zst") || ends_with(path, ".zst.tmp");
+}
+
+struct TextSink::Impl {
+    FILE* f = nullptr;
+    bool compress = false;
+    ZSTD_CStream* zs = nullptr;
+    std::vector<char> buf, zout;
+    size_t used = 0;
+    std::string path;
+
+    void write_raw(const void* data, size_t n) {
+        if (n && std::fwrite(data, 1, n, f) != n)
+            throw std::runtime_error("failed while writing " + path);
+    }
+    void flush_buf() {
+        if (!used) return;
+        if (!compress) {
+            write_raw(buf.data(), used);
+        } else {
+            ZSTD_inBuffer in{buf.data(), used, 0};
+            while (in.pos < in.size) {
+                ZSTD_outBuffer out{zout.data(), zout.size(), 0};
+                size_t rc = ZSTD_compressStream(zs, &out, &in);
+                if (ZSTD_isError(rc)) throw std::runtime_error(std::string("zstd: ") + ZSTD_getErrorName(rc));
+                write_raw(zout.data(), out.pos);
+            }
+        }
+        used = 0;
+    }
+};
+
+TextSink::TextSink(const std::string& path, bool compress) : impl_(new Impl) {
+    impl_->path = path;
+    impl_->compress = compress;
+    impl_->f = std::fopen(path.c_str(), "wb");
+    if (!impl_->f) throw std::runtime_error("failed to open " + path);
+    impl_->buf.resize(1 << 22);
+    if (compress) {
+        impl_->zs = ZSTD_createCStream();
+        if (!impl_->zs) throw std::runtime_error("failed to allocate a zstd stream");
+        size_t rc = ZSTD_initCStream(impl_->zs, 3);
+        if (ZSTD_isError(rc)) throw std::runtime_error(std::string("zstd init: ") + ZSTD_getErrorName(rc));
+        impl_->zout.resize(ZSTD_CStreamOutSize());
+    }
+}
+
+TextSink::~TextSink() {
+    if (impl_->zs) ZSTD_freeCStream(impl_->zs);
+    if (impl_->f) std::fclose(impl_->f);
+}
+
+void TextSink::put(const char* data, size_t n) {
+    while (n) {
+        size_t room = impl_->buf.size() - impl_->used;
+        size_t take = n < room ? n : room;
+        std::memcpy(impl_->buf.data() + impl_->used, data, take);
+        impl_->used += take;
+        data += take;
+        n -= take;
+        if (impl_->used == impl_->buf.size()) impl_->flush_buf();
+    }
+}
+
+void TextSink::put_u64(uint64_t v) {
+    char tmp[24];
+    auto r = std::to_chars(tmp, tmp + sizeof(tmp), v);
+    put(tmp, (size_t)(r.ptr - tmp));
+}
+
+void TextSink::put_char(char c) { put(&c, 1); }
+
+void TextSink::finish() {
+    impl_->flush_buf();
+    if (impl_->compress) {
+        for (;;) {
+            ZSTD_outBuffer out{impl_->zout.data(), impl_->zout.size(), 0};
+            size_t left = ZSTD_endStream(impl_->zs, &out);
+            if (ZSTD_isError(left)) throw std::runtime_error(std::string("zstd end: ") + ZSTD_getErrorName(left));
+            impl_->write_raw(impl_->zout.data(), out.pos);
+            if (left == 0) break;
+        }
+    }
+    if (std::fclose(impl_->f) != 0) {
+        impl_->f = nullptr;
+        throw std::runtime_error("failed to close " + impl_->path);
+    }
+    impl_->f = nullptr;
+}
+
+std::string read_maybe_compressed(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("failed to open " + path);
+    std::ostringstream ss;
+    ss << in.rdbuf();
+    std::string raw = ss.str();
+    static const unsigned char magic[4] = {0x28, 0xb5, 0x2f, 0xfd};
+    if (raw.size() < 4 || std::memcmp(raw.data(), magic, 4) != 0) return raw;
+    ZSTD_DStream* ds = ZSTD_createDStream();
+    if (!ds) throw std::runtime_error("failed to allocate a zstd decompression stream");
+    ZSTD_initDStream(ds);
+    std::string text;
+    std::vector<char> out(ZSTD_DStreamOutSize());
+    ZSTD_inBuffer zin{raw.data(), raw.size(), 0};
+    while (zin.pos < zin.size) {
+        ZSTD_outBuffer zo{out.data(), out.size(), 0};
+        size_t rc = ZSTD_decompressStream(ds, &zo, &zin);
+        if (ZSTD_isError(rc)) {
+            ZSTD_freeDStream(ds);
+            throw std::runtime_error(std::string("zstd: ") + ZSTD_getErrorName(rc));
+        }
+        text.append(out.data(), zo.pos);
+    }
+    for (;;) {  // drain buffered output
+        ZSTD_outBuffer zo{out.data(), out.size(), 0};
+        size_t rc = ZSTD_decompressStream(ds, &zo, &zin);
+        if (ZSTD_isError(rc)) break;
+        text.append(out.data(), zo.pos);
+        if (zo.pos == 0) break;
+    }
+    ZSTD_freeDStream(ds);
+    return text;
+}
+
+static void put_line_strided(TextSink& s, const uint32_t* v, uint64_t count, uint64_t stride,
+                             uint32_t p) {
+    for (uint64_t i = 0; i < count; ++i) {
+        if (i) s.put_char(' ');
+        s.put_u64(v[i * stride] % p);
+    }
+    s.put_char('\n');
+}
+
+void save_wdm_sym(const std::string& path, uint64_t rows, uint64_t cols, uint32_t p,
+                  const uint32_t* dr, const uint32_t* dc, const uint32_t* v, const uint32_t* curv,
+                  uint32_t b, const uint32_t* seq, uint64_t L) {
+    if (L == 0) throw std::runtime_error("Cannot save WDM file with empty sequence list");
+    const std::string tmp = path + ".tmp";
+    {
+        TextSink s(tmp, path_requests_compression(tmp));
+        s.put_u64(rows); s.put_char(' ');
+        s.put_u64(cols); s.put_char(' ');
+        s.put_u64(p); s.put_char(' ');
+        s.put_u64(L); s.put_char(' ');
+        s.put_u64(b); s.put_char('\n');
+        // preconditioners are written as stored (cuda/wdmfiles.h:195-199)
+        for (uint64_t i = 0; i < rows; ++i) { if (i) s.put_char(' '); s.put_u64(dr[i]); }
+        s.put_char('\n');
+        for (uint64_t i = 0; i < cols; ++i) { if (i) s.put_char(' '); s.put_u64(dc[i]); }
+        s.put_char('\n');
+        for (uint32_t c = 0; c < b; ++c) put_line_strided(s, v + c, cols, b, p);
+        for (uint32_t c = 0; c < b; ++c) put_line_strided(s, curv + c, cols, b, p);
+        const uint64_t ut = (uint64_t)b * (b + 1) / 2;
+        for (uint64_t e = 0; e < ut; ++e) put_line_strided(s, seq + e, L, ut, p);
+        s.finish();
+    }
+    std::filesystem::rename(tmp, path);
+}
+
+namespace {
+struct Scanner {
+    const std::string& t;
+    size_t pos = 0;
+    explicit Scanner(const std::string& text) : t(text) {}
+    uint64_t next(const char* what) {
+        while (pos < t.size() && (t[pos] == ' ' || t[pos] == '\n' || t[pos] == '\r' || t[pos] == '\t')) ++pos;
+        if (pos >= t.size() || t[pos] < '0' || t[pos] > '9')
+            throw std::runtime_error(std::string("Failed to read ") + what + " from WDM file");
+        uint64_t v = 0;
+        while (pos < t.size() && t[pos] >= '0' && t[pos] <= '9') v = v * 10 + (uint64_t)(t[pos++] - '0');
+        return v;
+    }
+};
+
+void read_initial(Scanner& sc, WdmInitial& w) {
+    w.rows = sc.next("matrix row count");
+    w.cols = sc.next("matrix column count");
+    w.prime = (uint32_t)sc.next("prime");
+    w.length = sc.next("sequence length");
+    w.block = (uint32_t)sc.next("vector count");
+    w.dr.resize(w.rows);
+    for (auto& x : w.dr) x = (uint32_t)sc.next("row preconditioner entry");
+    w.dc.resize(w.cols);
+    for (auto& x : w.dc) x = (uint32_t)sc.next("column preconditioner entry");
+    w.v.resize(w.cols * w.block);
+    for (uint32_t c = 0; c < w.block; ++c)
+        for (uint64_t r = 0; r < w.cols; ++r) w.v[r * w.block + c] = (uint32_t)sc.next("initial vector entry");
+}
+}  // namespace
+
+WdmInitial load_wdm_initial(const std::string& path) {
+    std::string text = read_maybe_compressed(path);
+    Scanner sc(text);
+    WdmInitial w;
+    read_initial(sc, w);
+    return w;
+}
+
+WdmFull load_wdm_full(const std::string& path) {
+    std::string text = read_maybe_compressed(path);
+    Scanner sc(text);
+    WdmFull w;
+    read_initial(sc, w);
+    w.curv.resize(w.cols * w.block);
+    for (uint32_t c = 0; c < w.block; ++c)
+        for (uint64_t r = 0; r < w.cols; ++r) w.curv[r * w.block + c] = (uint32_t)sc.next("current vector entry");
+    const uint64_t ut = (uint64_t)w.block * (w.block + 1) / 2;
+    w.seq.resize(w.length * ut);
+    for (uint64_t e = 0; e < ut; ++e)
+        for (uint64_t t = 0; t < w.length; ++t) w.seq[t * ut + e] = (uint32_t)sc.next("sequence entry");
+    return w;
+}
+
+}  // namespace pgwio
