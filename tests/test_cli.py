@@ -128,7 +128,11 @@ def test_recovery_cli_certificate(tmp_path, g12, g12_reference):
     lines = (tmp_path / "k.wdm_nullvectors_2.txt").read_text().splitlines()
     got = np.array([list(map(int, ln.split())) for ln in lines]).T
     assert (got == K0).all()
-    gen.write_text("1 2 3 4\n")
-    r = run([REC, G12, "-f", wdm, "--sector", sector, "--preconditioned-output", tmp_path / "x1",
-             "--original-output", tmp_path / "x2", "--metadata-output", tmp_path / "x3"])
+    # an initial block outside the kernel fails the certificate
+    bad = tmp_path / "bad.wdm"
+    W = np.random.default_rng(0).integers(1, p, size=V.shape).astype(np.uint32)
+    oc.lib().orc_write_wdm(str(bad).encode(), o.rows, o.cols, p, dr, dc, W.reshape(-1), W.reshape(-1),
+                           b, seq, 1)
+    (tmp_path / "bad_generators.txt").write_text("1 0 0 1\n")
+    r = run([REC, G12, "-f", bad, "--sector", sector, "--rowmix-rounds", "2"])
     assert r.returncode == 1 and "certificate failed" in r.stderr

@@ -66,7 +66,7 @@ def gpu_apply(torch, op, x):
     d_out = torch.empty_like(d_in)
     op.apply_square(d_in.data_ptr(), d_out.data_ptr(), stream(torch))
     torch.cuda.synchronize()
-    return host(d_out)
+    return host(d_out).reshape(-1)
 
 
 # ------------------------------------------------------------ cyclic (g12) --
@@ -148,7 +148,7 @@ def test_small_synthetic_plain_and_mfma_vs_oracle(torch_cuda, ctx, p, b):
 
 def test_odd_rows_and_plain_only_block_widths(torch_cuda, ctx):
     # N odd (one row left out of every round's pairs), b = 3 runs on the plain path
-    n, k, M, Mp, p, b = 5, 2, 3, 4, 1000003, 3
+    n, k, M, Mp, p, b = 7, 3, 3, 4, 1000003, 3  # N = 35 * 3 = 105
     spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=5)
     assert N % 2 == 1
     o = oc.synthetic_op(n, k, M, Mp, p, T, E, dr, dc, 2, pairs.reshape(-1))
