@@ -30,9 +30,10 @@ P31 = 2147483647
 WORKLOAD = dict(n=20, k=5, M=200, M_prime=220, prime=P31, block=16, table_seed=0, seed=2)
 
 
-def macs_per_sweep(n, k, M, Mp):
-    # every output panel entry sums (incidences) x (input panel rows) products
-    return comb(n, k - 1) * (n - k + 1) * Mp * M
+def macs_per_sweep(n, k, M, Mp, b):
+    # every entry of every output panel (b columns) sums incidences x input-panel-rows
+    # products: C(n,k-1) * (n-k+1) * M' * M * b, the same for all four sweeps
+    return comb(n, k - 1) * (n - k + 1) * Mp * M * b
 
 
 def cpu_baseline(spec, T, E, dr, dc, pairs, v, b, budget_s=20.0):
@@ -101,6 +102,8 @@ def main():
     x = torch.from_numpy(v.view(np.int32).copy()).cuda()
     ya, yb = torch.empty_like(x), torch.empty_like(x)
     torch.cuda.synchronize()
+    # the autotuner picks the MFMA tile configuration of each sweep (untimed)
+    tile_choice, tile_ms = op.autotune(3)
 
     # bit-exactness gate: one apply on each path from the same input
     op.apply_square(x.data_ptr(), ya.data_ptr(), None)
@@ -138,13 +141,14 @@ def main():
     plain_rate = 1000.0 / (plain_ms / max(1, args.plain_steps))
 
     nl = 4 if p > 65536 else 2
-    ops_per_sweep = 2.0 * macs_per_sweep(n, k, M, Mp) * nl * nl  # int8 ops, algorithmic
+    ops_per_sweep = 2.0 * macs_per_sweep(n, k, M, Mp, b) * nl * nl  # int8 ops, algorithmic
     achieved = ops_per_sweep / (sweep_ms * 1e-3) / 1e12
     peak = 5000.0  # TOP/s dense int8, MI355X
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None,
                 "kernel": "k_dense_mfma", "sweep_ms": round(sweep_ms, 4),
-                "sweeps_per_apply": sweeps}
+                "sweeps_per_apply": sweeps, "tile_configs": tile_choice,
+                "tile_ms": [round(t, 4) for t in tile_ms]}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -188,6 +188,14 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
 int pgw_kernel_vectors(pgw_op* op, const uint32_t* h_v, const uint32_t* h_gens, uint64_t ngen,
                        uint32_t* h_d, uint32_t* h_product);
 
+/* Autotuner: times every MFMA tile configuration of each of the four sweeps
+ * (`reps` launches each, HIP events) and keeps the fastest for this operator.
+ * h_choice[4] / h_ms[4] (optional) receive the choices and their per-launch
+ * milliseconds.  No counterpart in the reference (its kernels have one
+ * fixed launch shape, cuda/cyclic_cuda_helpers.h:455-489); results do not
+ * depend on the choice. */
+int pgw_operator_autotune(pgw_op* op, uint32_t reps, uint32_t* h_choice, double* h_ms);
+
 /* Timing of the hot path on the operator's own stream with HIP events:
  * `iters` block-applies d_out = S d_in (ping-pong), total milliseconds and
  * the per-launch average of the dominant sweep kernel (ms). */

@@ -80,7 +80,7 @@ EXPORTED = [
     "pgw_rowmix_realized_seed", "pgw_rowmix_plan", "pgw_rowmix_plan_hash", "pgw_fnv1a64_u32",
     "pgw_synthetic_table", "pgw_operator_dims", "pgw_operator_create", "pgw_operator_destroy",
     "pgw_operator_info", "pgw_apply_square", "pgw_apply_forward", "pgw_gram_upper",
-    "pgw_sequence_run", "pgw_kernel_vectors", "pgw_bench_apply",
+    "pgw_sequence_run", "pgw_kernel_vectors", "pgw_bench_apply", "pgw_operator_autotune",
 ]
 
 
@@ -127,6 +127,7 @@ def lib():
         "pgw_kernel_vectors": (I, [vp, vp, vp, u64, vp, vp]),
         "pgw_bench_apply": (I, [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u32)]),
+        "pgw_operator_autotune": (I, [vp, u32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -353,6 +354,13 @@ class Operator:
         _check(lib().pgw_bench_apply(self.h, d_a, d_b, iters, ctypes.byref(total),
                                      ctypes.byref(sweep), ctypes.byref(count)))
         return total.value, sweep.value, count.value
+
+    def autotune(self, reps: int = 3):
+        """Pick the fastest MFMA tile configuration per sweep; returns (choices, ms)."""
+        choice = np.zeros(4, np.uint32)
+        ms = np.zeros(4, np.float64)
+        _check(lib().pgw_operator_autotune(self.h, reps, _ptr(choice), _ptr(ms)))
+        return choice.tolist(), ms.tolist()
 
     # -- host-buffer entry points ------------------------------------------------
     def sequence(self, v, max_len: int, progress=None):

@@ -117,6 +117,7 @@ struct pgw_op {
     uint32_t* d_tab[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     uint32_t k4[4] = {0, 0, 0, 0};
     MfmaTable mt[4];
+    int mfma_cfg[4] = {1, 1, 1, 1};  // tile configuration per sweep (pgw_operator_autotune)
     uint32_t nl = 0;
     int8_t* d_lp_src = nullptr;
     int8_t* d_lp_mid = nullptr;
@@ -149,6 +150,7 @@ DenseSweep sweep_desc(const pgw_op* op, int which) {
     s.v_in = s.fwd ? op->u_src : op->u_tgt;
     s.r_out = s.fwd ? op->Mp : op->M;
     s.k_in = s.fwd ? op->M : op->Mp;
+    s.binom = op->ctx->d_binom;
     return s;
 }
 
@@ -186,7 +188,8 @@ void run_sweep(pgw_op* op, int which, const uint32_t* in_u32, const int8_t* in_l
     const DenseSweep s = sweep_desc(op, which);
     SweepTimer t(op, st);
     if (op->method == PGW_METHOD_MFMA)
-        launch_dense_mfma(s, op->mt[which], in_lp, out, scale, op->b, op->nl, op->m, st);
+        launch_dense_mfma(s, op->mt[which], in_lp, out, scale, op->b, op->nl, op->m, st,
+                          op->mfma_cfg[which]);
     else
         launch_dense_plain(s, op->d_tab[which][0], op->d_tab[which][1], op->k4[which], in_u32, out,
                            scale, op->b, op->m, st);
@@ -290,6 +293,22 @@ int pgw_ctx_create(int device, pgw_ctx** out) {
             throw Error(PGW_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
         }
         upload_binomials(device);
+        std::vector<uint64_t> binom(64 * 64, 0);
+        for (uint32_t a = 0; a < 64; ++a)
+            for (uint32_t b = 0; b <= a; ++b) {
+                unsigned __int128 v = 1;
+                const uint32_t kk = std::min(b, a - b);
+                for (uint32_t q = 0; q < kk; ++q) v = v * (a - q) / (q + 1);
+                binom[a * 64 + b] = v > (unsigned __int128)UINT64_MAX ? UINT64_MAX : (uint64_t)v;
+            }
+        e = hipMalloc(&c->d_binom, binom.size() * sizeof(uint64_t));
+        if (e == hipSuccess)
+            e = hipMemcpy(c->d_binom, binom.data(), binom.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            throw Error(PGW_ERR_HIP, std::string("binomial table upload failed: ") + hipGetErrorString(e));
+        }
         *out = c;
     });
 }
@@ -300,6 +319,7 @@ int pgw_ctx_destroy(pgw_ctx* ctx) {
         PGW_HIP(hipSetDevice(ctx->device));
         PGW_HIP(hipStreamSynchronize(ctx->stream));
         PGW_HIP(hipStreamDestroy(ctx->stream));
+        if (ctx->d_binom) (void)hipFree(ctx->d_binom);
         delete ctx;
     });
 }
@@ -385,6 +405,11 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             const uint32_t* src[4] = {desc->table_d, desc->table_e, desc->table_e, desc->table_d};
             if (method == PGW_METHOD_MFMA) {
                 op->nl = mfma_limbs(op->p);
+                if (const char* env = std::getenv("PGW_MFMA_CFG")) {
+                    const int c = std::atoi(env);
+                    require(c >= 0 && c < mfma_config_count(), "PGW_MFMA_CFG out of range");
+                    for (int& x : op->mfma_cfg) x = c;
+                }
                 for (int w = 0; w < 4; ++w) {
                     DenseSweep s = sweep_desc(op, w);
                     check_mfma_bound(s, op->nl);
@@ -629,6 +654,54 @@ int pgw_kernel_vectors(pgw_op* op, const uint32_t* h_v, const uint32_t* h_gens, 
     for (uint32_t* p : {d_B, d_N, d_D, d_G})
         if (p) (void)hipFree(p);
     return rc;
+}
+
+int pgw_operator_autotune(pgw_op* op, uint32_t reps, uint32_t* h_choice, double* h_ms) {
+    return guarded([&] {
+        require(op != nullptr, "null operator");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        hipStream_t st = op->ctx->stream;
+        const int count = mfma_config_count();
+        if (op->method != PGW_METHOD_MFMA) {
+            for (int w = 0; w < 4; ++w) {
+                if (h_choice) h_choice[w] = 0;
+                if (h_ms) h_ms[w] = 0.0;
+            }
+            return;
+        }
+        if (reps == 0) reps = 1;
+        hipEvent_t e0, e1;
+        PGW_HIP(hipEventCreate(&e0));
+        PGW_HIP(hipEventCreate(&e1));
+        // the sweeps' schedules do not depend on the data: time them on the
+        // operator's own digit planes and scratch blocks
+        for (int w = 0; w < 4; ++w) {
+            const DenseSweep s = sweep_desc(op, w);
+            const int8_t* lp = s.fwd ? op->d_lp_src : op->d_lp_mid;
+            uint32_t* out = s.fwd ? op->d_mid : op->d_target;
+            double best = 1e30;
+            int best_cfg = op->mfma_cfg[w];
+            for (int c = 0; c < count; ++c) {
+                launch_dense_mfma(s, op->mt[w], lp, out, nullptr, op->b, op->nl, op->m, st, c);  // warm
+                PGW_HIP(hipEventRecord(e0, st));
+                for (uint32_t r = 0; r < reps; ++r)
+                    launch_dense_mfma(s, op->mt[w], lp, out, nullptr, op->b, op->nl, op->m, st, c);
+                PGW_HIP(hipEventRecord(e1, st));
+                PGW_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                PGW_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / reps < best) {
+                    best = ms / reps;
+                    best_cfg = c;
+                }
+            }
+            op->mfma_cfg[w] = best_cfg;
+            if (h_choice) h_choice[w] = (uint32_t)best_cfg;
+            if (h_ms) h_ms[w] = best;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
 }
 
 int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, double* total_ms,

@@ -150,6 +150,7 @@ struct HostBinom {
 struct pgw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    uint64_t* d_binom = nullptr;  // C(a,b), a,b < 64, row-major
 };
 
 namespace pgw {

@@ -20,6 +20,7 @@ struct DenseSweep {
     uint64_t v_in = 0;        // input subsets
     uint32_t r_out = 0;       // output rows per subset panel
     uint32_t k_in = 0;        // input rows per subset panel
+    const uint64_t* binom = nullptr;  // device C(a,b) table, 64 x 64 (MFMA path)
 };
 
 // plain u64 path ------------------------------------------------------------
@@ -73,8 +74,10 @@ uint64_t limb_plane_bytes(uint64_t v, uint32_t k_in, uint32_t b, uint32_t nl);
 uint32_t limb_stride(uint32_t k_in);
 void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
                        uint32_t k_in, uint32_t b, uint32_t nl, const ModP& m, hipStream_t st);
+// cfg selects a tile configuration in [0, mfma_config_count())
+int mfma_config_count();
 void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
                        const uint32_t* scale, uint32_t b, uint32_t nl, const ModP& m,
-                       hipStream_t st);
+                       hipStream_t st, int cfg);
 
 }  // namespace pgw

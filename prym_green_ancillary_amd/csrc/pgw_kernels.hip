@@ -646,341 +646,4 @@ void launch_accumulate_generator(const uint32_t* Bm, const uint32_t* G, uint32_t
     PGW_CHECK_LAUNCH("accumulate generator");
 }
 
-// ===========================================================================
-// int8-limb MFMA table contraction
-// ===========================================================================
-//
-// Residue x in [0,p) is written as a balanced base-256 number
-//   x == y = sum_j d_j 256^j (mod p),  d_j in [-128, 127],  j < NL,
-// with y = x if x <= MAXPOS else x - p (MAXPOS = 127*(256^NL-1)/255); this is
-// possible for every residue because p <= 256^NL.  A product of two residues is
-// then sum_{j,l} d_j e_l 256^(j+l); the MFMA accumulates, per digit shift
-// s = j + l, the int32 sum of |d_j e_l| <= 2^14 over (incidences x K) terms,
-// so an accumulator stays below 2^31 while incidences * K * NL < 2^17 (checked
-// on the host).  The epilogue folds the shifts with w_s = 256^s mod p.
-//
-// Digit planes of an input block: lp[sign][v][limb][c][k] (k fastest, row
-// stride K_stride bytes, 16-byte aligned), sign 1 holding the digits of -x mod p
-// so that the incidence sign is applied by choosing the plane.  Tables:
-// tabl[i][limb][row][k] with zero rows/columns up to (R_pad, K_pad); a B
-// fragment may read past K_in into the next column's bytes, which meet zero
-// table entries and contribute nothing.
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-// compile-time loop: f(integral_constant<int, I>) for I = 0..N-1, so that
-// register-array indices are constants after inlining
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-uint32_t mfma_limbs(uint32_t p) { return p <= 65536u ? 2u : 4u; }
-bool mfma_block_supported(uint32_t b) { return b == 4 || b == 8 || b == 16; }
-
-static inline uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
-uint32_t limb_stride(uint32_t k_in) { return round_up(k_in, 16); }
-
-uint64_t limb_plane_bytes(uint64_t v, uint32_t k_in, uint32_t b, uint32_t nl) {
-    // + slack for fragment reads past the last row (at most K_pad - K_stride bytes)
-    return 2ULL * v * nl * b * limb_stride(k_in) + 4096;
-}
-
-struct DigitSpec {
-    int64_t maxpos;
-    uint64_t bias;
-};
-
-__host__ __device__ inline DigitSpec digit_spec(uint32_t nl) {
-    return nl == 2 ? DigitSpec{32639, 0x8080ULL} : DigitSpec{2139062143LL, 0x80808080ULL};
-}
-
-// bytes d_j + 128 of the representative, as unsigned; caller XORs 0x80 per byte
-__host__ __device__ inline uint64_t digits_biased(uint32_t x, uint32_t p, const DigitSpec& ds) {
-    int64_t y = (int64_t)x <= ds.maxpos ? (int64_t)x : (int64_t)x - (int64_t)p;
-    return (uint64_t)(y + (int64_t)ds.bias);
-}
-
-void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, uint32_t p,
-                      const uint32_t* src, bool transpose_src) {
-    // src[i][a][b]: if !transpose_src rows = a-index (dim rows), k = b-index (k_in);
-    // if transpose_src the table is read as src[i][k][row].
-    const uint32_t nl = mfma_limbs(p);
-    t.rows = rows;
-    t.k_in = k_in;
-    t.rows_pad = round_up(rows, 64);
-    t.k_pad = round_up(k_in, 64);
-    const DigitSpec ds = digit_spec(nl);
-    std::vector<int8_t> host((size_t)n * nl * t.rows_pad * t.k_pad, 0);
-    for (uint32_t i = 0; i < n; ++i)
-        for (uint32_t r = 0; r < rows; ++r)
-            for (uint32_t k = 0; k < k_in; ++k) {
-                uint32_t v = transpose_src ? src[((size_t)i * k_in + k) * rows + r]
-                                           : src[((size_t)i * rows + r) * k_in + k];
-                uint64_t u = digits_biased(v, p, ds);
-                for (uint32_t j = 0; j < nl; ++j)
-                    host[(((size_t)i * nl + j) * t.rows_pad + r) * t.k_pad + k] =
-                        (int8_t)(uint8_t)(((u >> (8 * j)) & 0xff) ^ 0x80);
-            }
-    PGW_HIP(hipMalloc(&t.d, host.size()));
-    PGW_HIP(hipMemcpy(t.d, host.data(), host.size(), hipMemcpyHostToDevice));
-}
-
-// one thread per (panel v, 16-row group of k, column c): reads 16 words of a
-// column, writes 16 digit bytes per limb and sign (one 16-byte store each)
-template <int B, int NL>
-__global__ __launch_bounds__(256) void k_limb_split(const uint32_t* __restrict__ in,
-                                                    const uint32_t* __restrict__ scale,
-                                                    int8_t* __restrict__ lp, uint64_t v_count,
-                                                    uint32_t k_in, uint32_t k_stride, ModP m) {
-    const uint32_t groups = k_stride / 16;
-    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t total = v_count * groups * B;
-    if (tid >= total) return;
-    const int c = (int)(tid % B);
-    const uint64_t rest = tid / B;
-    const uint32_t kg = (uint32_t)(rest % groups);
-    const uint64_t v = rest / groups;
-    const DigitSpec ds = digit_spec(NL);
-    uint32_t w[2][NL][4];
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int j = 0; j < NL; ++j)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) w[s][j][q] = 0;
-#pragma unroll
-    for (int tt = 0; tt < 16; ++tt) {
-        const uint32_t k = kg * 16 + tt;
-        uint32_t x = 0;
-        if (k < k_in) {
-            const uint64_t row = v * k_in + k;
-            x = in[row * B + c];
-            if (scale) x = mulmod(x, scale[row], m);
-        }
-        const uint32_t xn = x ? m.p - x : 0u;
-        const uint64_t u0 = digits_biased(x, m.p, ds), u1 = digits_biased(xn, m.p, ds);
-#pragma unroll
-        for (int j = 0; j < NL; ++j) {
-            w[0][j][tt / 4] |= (uint32_t)((((u0 >> (8 * j)) & 0xff) ^ 0x80) << (8 * (tt % 4)));
-            w[1][j][tt / 4] |= (uint32_t)((((u1 >> (8 * j)) & 0xff) ^ 0x80) << (8 * (tt % 4)));
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int j = 0; j < NL; ++j) {
-            int8_t* dst = lp + ((((uint64_t)s * v_count + v) * NL + j) * B + c) * k_stride + kg * 16;
-            *reinterpret_cast<uint4*>(dst) = make_uint4(w[s][j][0], w[s][j][1], w[s][j][2], w[s][j][3]);
-        }
-}
-
-void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
-                       uint32_t k_in, uint32_t b, uint32_t nl, const ModP& m, hipStream_t st) {
-    const uint32_t ks = limb_stride(k_in);
-    const uint64_t total = v * (ks / 16) * b;
-    if (total == 0) return;
-    dim3 grid(grid_for(total)), blk(256);
-#define PGW_LS(BB, NN) hipLaunchKernelGGL((k_limb_split<BB, NN>), grid, blk, 0, st, in, scale, lp, v, k_in, ks, m)
-    if (nl == 2) {
-        if (b == 4) PGW_LS(4, 2); else if (b == 8) PGW_LS(8, 2); else if (b == 16) PGW_LS(16, 2);
-        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
-    } else {
-        if (b == 4) PGW_LS(4, 4); else if (b == 8) PGW_LS(8, 4); else if (b == 16) PGW_LS(16, 4);
-        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
-    }
-#undef PGW_LS
-    PGW_CHECK_LAUNCH("limb split");
-}
-
-struct ShiftWeights {
-    uint32_t w[8];
-};
-
-// One wavefront owns RT row tiles (16 rows each) x CT column tiles (16 columns
-// = 16/B subset panels each) of the output and iterates the n tables; a tile's
-// MFMAs for table i run only if some panel of the tile is incident to i.
-template <int B, int NL, int RT, int CT, bool FWD>
-__global__ __launch_bounds__(256) void k_dense_mfma(
-    const int8_t* __restrict__ tabl, const int8_t* __restrict__ lp, uint32_t n, uint32_t deg_out,
-    uint64_t u_out, uint64_t v_in, uint32_t r_out, uint32_t r_pad, uint32_t k_pad,
-    uint32_t k_stride, uint32_t row_slices, uint64_t total_waves, uint32_t* __restrict__ out,
-    const uint32_t* __restrict__ scale, ModP m, ShiftWeights sw) {
-    constexpr int NS = 2 * NL - 1;
-    constexpr int SPT = 16 / B;        // subset panels per column tile
-    constexpr int GS = CT * SPT;       // subset panels per wave
-    constexpr int MAXN = 32;
-    __shared__ uint64_t s_mask[4][GS];
-    __shared__ uint32_t s_code[4][GS * MAXN];
-    const int wave = threadIdx.x / 64;
-    const int lane = threadIdx.x % 64;
-    const uint64_t wid = (uint64_t)blockIdx.x * 4 + wave;
-    const bool live = wid < total_waves;
-    const uint64_t cg = live ? wid / row_slices : 0;
-    const uint32_t rs = live ? (uint32_t)(wid % row_slices) : 0;
-    const uint32_t row0 = rs * 16 * RT;
-    DeviceBinom C;
-    if (lane < GS) {
-        const uint64_t u = cg * GS + lane;
-        s_mask[wave][lane] = u < u_out ? unrank_mask(u, (int)n, (int)deg_out, C) : 0ULL;
-    }
-    __syncthreads();
-    for (int idx = lane; idx < GS * (int)n; idx += 64) {
-        const int g = idx / (int)n, i = idx % (int)n;
-        const uint64_t u = cg * GS + g;
-        uint32_t code = 0xffffffffu;
-        if (u < u_out) {
-            const uint64_t mask = s_mask[wave][g];
-            const bool bit = (mask >> i) & 1ULL;
-            if (FWD ? !bit : bit) {
-                const uint32_t neg = (uint32_t)__popcll(mask & ((1ULL << i) - 1)) & 1u;
-                const uint64_t nb = FWD ? rank_mask(mask | (1ULL << i), (int)n, (int)deg_out + 1, C)
-                                        : rank_mask(mask & ~(1ULL << i), (int)n, (int)deg_out - 1, C);
-                code = (uint32_t)nb | (neg << 31);
-            }
-        }
-        s_code[wave][g * MAXN + i] = code;
-    }
-    __syncthreads();
-    if (!live) return;
-
-    v4i acc[RT][CT][NS];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) acc[rt][ct][s] = v4i{0, 0, 0, 0};
-
-    const int cc = lane % 16;          // MFMA column of this lane
-    const int kq = (lane / 16) * 16;   // byte offset of this lane's K slice
-    const int my_slot = cc / B, my_c = cc % B;
-    const uint64_t plane = (uint64_t)NL * B * k_stride;  // bytes per panel (one sign)
-
-    for (uint32_t i = 0; i < n; ++i) {
-        uint32_t code[CT];
-        bool on[CT];
-        bool any = false;
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-            code[ct] = s_code[wave][(ct * SPT + my_slot) * MAXN + i];
-            bool o = false;
-#pragma unroll
-            for (int q = 0; q < SPT; ++q) o |= s_code[wave][(ct * SPT + q) * MAXN + i] != 0xffffffffu;
-            on[ct] = o;
-            any |= o;
-        }
-        if (!any) continue;
-        const int8_t* abase = tabl + ((uint64_t)i * NL * r_pad + row0 + cc) * k_pad + kq;
-        const int8_t* bbase[CT];
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-            const uint32_t cd = code[ct];
-            bbase[ct] = cd == 0xffffffffu
-                            ? nullptr
-                            : lp + ((uint64_t)(cd >> 31) * v_in + (cd & 0x7fffffffu)) * plane +
-                                  (uint64_t)my_c * k_stride + kq;
-        }
-        for (uint32_t k0 = 0; k0 < k_pad; k0 += 64) {
-            v4i a[RT][NL];
-            static_for<RT>([&](auto rt) {
-                static_for<NL>([&](auto j) {
-                    a[rt][j] = *reinterpret_cast<const v4i*>(
-                        abase + ((uint64_t)j * r_pad + rt * 16) * k_pad + k0);
-                });
-            });
-            v4i bf[CT][NL];
-            static_for<CT>([&](auto ct) {
-                static_for<NL>([&](auto l) {
-                    bf[ct][l] = bbase[ct] ? *reinterpret_cast<const v4i*>(
-                                                bbase[ct] + (uint64_t)l * B * k_stride + k0)
-                                          : v4i{0, 0, 0, 0};
-                });
-            });
-            static_for<CT>([&](auto ct) {
-                if (!on[ct]) return;
-                static_for<RT>([&](auto rt) {
-                    static_for<NL>([&](auto j) {
-                        static_for<NL>([&](auto l) {
-                            acc[rt][ct][j + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                a[rt][j], bf[ct][l], acc[rt][ct][j + l], 0, 0, 0);
-                        });
-                    });
-                });
-            });
-        }
-    }
-
-    // epilogue: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile
-    const uint64_t off = (uint64_t)m.p << 32;  // multiple of p above 2^31
-    static_for<RT>([&](auto rt) {
-        static_for<CT>([&](auto ct) {
-            const uint64_t u = cg * GS + ct * SPT + my_slot;
-            static_for<4>([&](auto v) {
-                const uint32_t row = row0 + rt * 16 + (lane / 16) * 4 + v;
-                if (u >= u_out || row >= r_out) return;
-                uint64_t s_lo = 0, s_hi = 0;
-                static_for<NS>([&](auto s) {
-                    const int32_t a32 = acc[rt][ct][s][decltype(v)::value];
-                    const uint32_t r = mod_u64((uint64_t)((int64_t)a32 + (int64_t)off), m);
-                    const uint64_t prod = (uint64_t)r * sw.w[s];  // < 2^62
-                    if (s < 4) s_lo += prod; else s_hi += prod;   // 4 terms < 2^64
-                });
-                uint32_t val = addmod(mod_u64(s_lo, m), mod_u64(s_hi, m), m.p);
-                const uint64_t orow = u * r_out + row;
-                if (scale) val = mulmod(val, scale[orow], m);
-                out[orow * B + my_c] = val;
-            });
-        });
-    });
-}
-
-template <int B, int NL, int RT, int CT>
-static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
-                           const uint32_t* scale, const ModP& m, hipStream_t st) {
-    constexpr int GS = CT * (16 / B);
-    require(s.n <= 32, "MFMA sweep supports n <= 32");
-    require(t.rows_pad % (16 * RT) == 0, "table padding does not match the row tiling");
-    const uint32_t row_slices = t.rows_pad / (16 * RT);
-    const uint64_t col_groups = (s.u_out + GS - 1) / GS;
-    const uint64_t total_waves = col_groups * row_slices;
-    const uint64_t blocks = (total_waves + 3) / 4;
-    require(blocks < (1ULL << 31), "MFMA sweep grid too large");
-    ShiftWeights sw{};
-    uint64_t w = 1;
-    for (int i = 0; i < 2 * NL - 1; ++i) {
-        sw.w[i] = (uint32_t)w;
-        w = (w * 256) % m.p;
-    }
-    const uint32_t deg_out = s.fwd ? s.k - 1 : s.k;
-    const uint32_t ks = limb_stride(s.k_in);
-#define PGW_MF(F)                                                                              \
-    hipLaunchKernelGGL((k_dense_mfma<B, NL, RT, CT, F>), dim3((unsigned)blocks), dim3(256), 0, \
-                       st, t.d, lp, s.n, deg_out, s.u_out, s.v_in, s.r_out, t.rows_pad,         \
-                       t.k_pad, ks, row_slices, total_waves, out, scale, m, sw)
-    if (s.fwd) PGW_MF(true); else PGW_MF(false);
-#undef PGW_MF
-    PGW_CHECK_LAUNCH("MFMA sweep");
-}
-
-void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
-                       const uint32_t* scale, uint32_t b, uint32_t nl, const ModP& m,
-                       hipStream_t st) {
-    if (nl == 4) {
-        if (b == 16) dense_mfma_cfg<16, 4, 2, 2>(s, t, lp, out, scale, m, st);
-        else if (b == 8) dense_mfma_cfg<8, 4, 2, 2>(s, t, lp, out, scale, m, st);
-        else if (b == 4) dense_mfma_cfg<4, 4, 2, 2>(s, t, lp, out, scale, m, st);
-        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
-    } else {
-        if (b == 16) dense_mfma_cfg<16, 2, 2, 4>(s, t, lp, out, scale, m, st);
-        else if (b == 8) dense_mfma_cfg<8, 2, 2, 4>(s, t, lp, out, scale, m, st);
-        else if (b == 4) dense_mfma_cfg<4, 2, 2, 4>(s, t, lp, out, scale, m, st);
-        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
-    }
-}
-
 }  // namespace pgw

@@ -1,0 +1,132 @@
+// pgw_mfma_host.hip -- host side and digit-plane kernel of the int8-limb MFMA
+// table contraction: table digits, limb split, launch dispatch by block width.
+#include "pgw_mfma_sweep.h"
+
+namespace pgw {
+
+static inline uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+uint32_t mfma_limbs(uint32_t p) { return p <= 65536u ? 2u : 4u; }
+bool mfma_block_supported(uint32_t b) { return b == 4 || b == 8 || b == 16; }
+uint32_t limb_stride(uint32_t k_in) { return round_up(k_in, 16); }
+
+uint64_t limb_plane_bytes(uint64_t v, uint32_t k_in, uint32_t b, uint32_t nl) {
+    // + slack for fragment reads past the last row (at most K_pad - K_stride bytes)
+    return 2ULL * v * nl * b * limb_stride(k_in) + 4096;
+}
+
+void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, uint32_t p,
+                      const uint32_t* src, bool transpose_src) {
+    // !transpose_src: src[i][row][k] (rows x k_in); transpose_src: src[i][k][row]
+    const uint32_t nl = mfma_limbs(p);
+    t.rows = rows;
+    t.k_in = k_in;
+    t.rows_pad = round_up(rows, 16);
+    t.k_pad = round_up(k_in, 32);
+    const DigitSpec ds = digit_spec(nl);
+    std::vector<int8_t> host((size_t)n * nl * t.rows_pad * t.k_pad, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t r = 0; r < rows; ++r)
+            for (uint32_t k = 0; k < k_in; ++k) {
+                uint32_t v = transpose_src ? src[((size_t)i * k_in + k) * rows + r]
+                                           : src[((size_t)i * rows + r) * k_in + k];
+                uint64_t u = digits_biased(v, p, ds);
+                for (uint32_t j = 0; j < nl; ++j)
+                    host[(((size_t)i * nl + j) * t.rows_pad + r) * t.k_pad + k] =
+                        (int8_t)(uint8_t)(((u >> (8 * j)) & 0xff) ^ 0x80);
+            }
+    PGW_HIP(hipMalloc(&t.d, host.size()));
+    PGW_HIP(hipMemcpy(t.d, host.data(), host.size(), hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------
+// digit planes of a block
+// ---------------------------------------------------------------------------
+
+// one thread per (panel v, 16-row group kg, column c): 16 words of a column in,
+// 16 digit bytes per limb and sign out (one 16-byte store each)
+template <int B, int NL>
+__global__ __launch_bounds__(256) void k_limb_split(const uint32_t* __restrict__ in,
+                                                    const uint32_t* __restrict__ scale,
+                                                    int8_t* __restrict__ lp, uint64_t v_count,
+                                                    uint32_t k_in, uint32_t k_stride, ModP m) {
+    const uint32_t groups = k_stride / 16;
+    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t total = v_count * groups * B;
+    if (tid >= total) return;
+    const int c = (int)(tid % B);
+    const uint64_t rest = tid / B;
+    const uint32_t kg = (uint32_t)(rest % groups);
+    const uint64_t v = rest / groups;
+    const DigitSpec ds = digit_spec(NL);
+    uint32_t w[2][NL][4];
+    static_for<2>([&](auto s) {
+        static_for<NL>([&](auto j) { static_for<4>([&](auto q) { w[s][j][q] = 0; }); });
+    });
+    static_for<16>([&](auto tt) {
+        const uint32_t k = kg * 16 + tt;
+        uint32_t x = 0;
+        if (k < k_in) {
+            const uint64_t row = v * k_in + k;
+            x = in[row * B + c];
+            if (scale) x = mulmod(x, scale[row], m);
+        }
+        const uint32_t xn = x ? m.p - x : 0u;
+        const uint64_t u0 = digits_biased(x, m.p, ds), u1 = digits_biased(xn, m.p, ds);
+        static_for<NL>([&](auto j) {
+            w[0][j][tt / 4] |= (uint32_t)((((u0 >> (8 * j)) & 0xff) ^ 0x80) << (8 * (tt % 4)));
+            w[1][j][tt / 4] |= (uint32_t)((((u1 >> (8 * j)) & 0xff) ^ 0x80) << (8 * (tt % 4)));
+        });
+    });
+    static_for<2>([&](auto s) {
+        static_for<NL>([&](auto j) {
+            int8_t* dst = lp + ((((uint64_t)s * v_count + v) * NL + j) * B + c) * k_stride + kg * 16;
+            *reinterpret_cast<uint4*>(dst) = make_uint4(w[s][j][0], w[s][j][1], w[s][j][2], w[s][j][3]);
+        });
+    });
+}
+
+static inline unsigned grid_for(uint64_t threads, unsigned block = 256) {
+    uint64_t g = (threads + block - 1) / block;
+    return (unsigned)std::max<uint64_t>(1, g);
+}
+
+void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
+                       uint32_t k_in, uint32_t b, uint32_t nl, const ModP& m, hipStream_t st) {
+    const uint32_t ks = limb_stride(k_in);
+    const uint64_t total = v * (ks / 16) * b;
+    if (total == 0) return;
+    dim3 grid(grid_for(total)), blk(256);
+#define PGW_LS(BB, NN) hipLaunchKernelGGL((k_limb_split<BB, NN>), grid, blk, 0, st, in, scale, lp, v, k_in, ks, m)
+    if (nl == 2) {
+        if (b == 4) PGW_LS(4, 2); else if (b == 8) PGW_LS(8, 2); else if (b == 16) PGW_LS(16, 2);
+        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
+    } else {
+        if (b == 4) PGW_LS(4, 4); else if (b == 8) PGW_LS(8, 4); else if (b == 16) PGW_LS(16, 4);
+        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
+    }
+#undef PGW_LS
+    PGW_CHECK_LAUNCH("limb split");
+}
+
+// ---------------------------------------------------------------------------
+int mfma_config_count() { return 3; }
+
+void launch_dense_mfma_b4(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
+                          uint32_t*, const uint32_t*, const ModP&, hipStream_t);
+void launch_dense_mfma_b8(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
+                          uint32_t*, const uint32_t*, const ModP&, hipStream_t);
+void launch_dense_mfma_b16(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
+                           uint32_t*, const uint32_t*, const ModP&, hipStream_t);
+
+void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
+                       const uint32_t* scale, uint32_t b, uint32_t nl, const ModP& m,
+                       hipStream_t st, int cfg) {
+    if (cfg < 0 || cfg >= mfma_config_count()) cfg = 1;
+    if (b == 16) launch_dense_mfma_b16(cfg, nl, s, t, lp, out, scale, m, st);
+    else if (b == 8) launch_dense_mfma_b8(cfg, nl, s, t, lp, out, scale, m, st);
+    else if (b == 4) launch_dense_mfma_b4(cfg, nl, s, t, lp, out, scale, m, st);
+    else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
+}
+
+}  // namespace pgw

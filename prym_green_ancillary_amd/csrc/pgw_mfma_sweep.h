@@ -1,0 +1,439 @@
+// pgw_mfma.hip -- the int8-limb MFMA table contraction of the dense Koszul sweep.
+//
+// Residue x in [0,p) is written as a balanced base-256 number
+//   x == y = sum_j d_j 256^j (mod p),  d_j in [-128, 127],  j < NL,
+// with y = x if x <= MAXPOS else x - p (MAXPOS = 127*(256^NL-1)/255); every
+// residue has such a representative because p <= 256^NL.  The product of two
+// residues is sum_{j,l} d_j e_l 256^(j+l): the MFMA accumulates, per digit
+// shift s = j + l, the int32 sum of products |d_j e_l| <= 2^14 over
+// (incidences x K) terms, so an accumulator stays below 2^31 while
+// incidences * K * NL < 2^17 (checked on the host).  The epilogue folds the
+// shifts with w_s = 256^s mod p and reduces exactly.
+//
+// Digit planes of an input block: lp[sign][v][limb][c][k] (k fastest, row
+// stride K_stride = round_up(K,16) bytes), sign 1 holding the digits of -x mod p
+// so that the incidence sign selects a plane.  Tables: tabl[i][limb][row][k]
+// with zero rows/columns up to (R_pad, K_pad), R_pad = round_up(R,16),
+// K_pad = round_up(K,32).  A B fragment may read past K into the next column's
+// bytes; those meet zero table entries and contribute nothing.
+//
+// Work decomposition: a workgroup of WR x WC wavefronts owns WR*RT row tiles
+// (16 rows each) x WC*CT column tiles (16 MFMA columns = 16/B subset panels) of
+// the output and walks the tables i = 0..n-1 that some panel of it is incident
+// to, in K chunks of 64 (32 for the tail).  The A (table) and B (digit plane)
+// tiles of a chunk are staged in LDS, double-buffered: the global loads of the
+// next chunk are in flight while the MFMAs of the current one run.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <utility>
+#include <vector>
+
+#include "pgw_kernels.h"
+
+namespace pgw {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, I>) for I = 0..N-1: register-array indices stay constants
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// binomial coefficients read from a 64 x 64 device table (one per context)
+struct GlobalBinom {
+    const uint64_t* t;
+    __device__ uint64_t operator()(int a, int b) const {
+        return (a < 0 || b < 0 || b > a) ? 0ULL : t[a * 64 + b];
+    }
+};
+
+struct DigitSpec {
+    int64_t maxpos;
+    uint64_t bias;
+};
+
+__host__ __device__ inline DigitSpec digit_spec(uint32_t nl) {
+    return nl == 2 ? DigitSpec{32639, 0x8080ULL} : DigitSpec{2139062143LL, 0x80808080ULL};
+}
+
+// bytes d_j + 128 of the representative; XOR 0x80 per byte gives the int8 digit
+__host__ __device__ inline uint64_t digits_biased(uint32_t x, uint32_t p, const DigitSpec& ds) {
+    int64_t y = (int64_t)x <= ds.maxpos ? (int64_t)x : (int64_t)x - (int64_t)p;
+    return (uint64_t)(y + (int64_t)ds.bias);
+}
+
+struct ShiftWeights {
+    uint32_t w[8];
+};
+
+struct MfmaSweepArgs {
+    const uint64_t* binom;  // C(a,b) table, 64 x 64
+    const int8_t* tabl;
+    const int8_t* lp;
+    uint32_t n, deg_out;
+    uint64_t u_out, v_in;
+    uint32_t r_out, r_pad, k_in, k_pad, k_stride;
+    uint32_t row_slices;
+    uint64_t col_groups;
+    uint32_t xcd_affinity;  // 1: grid is 8 * ceil(col_groups / (8 / row_slices)) blocks
+    uint32_t* out;
+    const uint32_t* scale;
+    ModP m;
+    ShiftWeights sw;
+};
+
+constexpr int kLdsStride = 64;  // bytes per staged row (one 64-wide K chunk)
+
+// Byte offset of (row, k) in a staged tile.  The four 16-byte pieces of a row
+// are rotated by row/4, so 16 consecutive rows read at one k land in 16
+// distinct 16-byte bank groups, and 4 consecutive whole rows written by 16
+// lanes cover the 64 banks once: both directions are conflict-free.
+__device__ __forceinline__ int lds_off(int row, int k) {
+    return row * kLdsStride + ((((k >> 4) + (row >> 2)) & 3) << 4) + (k & 15);
+}
+constexpr int kMaxN = 32;
+
+template <int B, int NL, int RT, int CT, int WR, int WC, bool FWD>
+__global__ __launch_bounds__(64 * WR * WC) void k_dense_mfma(MfmaSweepArgs a) {
+    constexpr int NW = WR * WC;
+    constexpr int NT = 64 * NW;
+    constexpr int NS = 2 * NL - 1;
+    constexpr int SPT = 16 / B;                 // subset panels per column tile
+    constexpr int TILES_C = WC * CT;            // column tiles per workgroup
+    constexpr int SLOTS = TILES_C * SPT;        // subset panels per workgroup
+    constexpr int ROWS = WR * RT * 16;          // rows per workgroup
+    constexpr int A_BYTES = NL * ROWS * kLdsStride;
+    constexpr int B_BYTES = TILES_C * NL * 16 * kLdsStride;
+    constexpr int A_ITEMS = NL * ROWS * 4;      // 16-byte pieces per 64-wide chunk
+    constexpr int B_ITEMS = TILES_C * NL * 16 * 4;
+    constexpr int A_PER = (A_ITEMS + NT - 1) / NT;
+    constexpr int B_PER = (B_ITEMS + NT - 1) / NT;
+
+    __shared__ __align__(16) int8_t s_a[2][A_BYTES];
+    __shared__ __align__(16) int8_t s_b[2][B_BYTES];
+    __shared__ uint64_t s_mask[SLOTS];
+    __shared__ uint32_t s_code[SLOTS * kMaxN];
+    __shared__ uint32_t s_any[kMaxN];
+
+    // XCD-aware order: hardware block ids are dealt round-robin to the 8 XCDs.
+    // When the row slices divide 8, XCD x works only on row slice x % R, so its
+    // L2 holds just that slice of the tables; its column groups are spread
+    // over the 8/R XCDs sharing the slice.  The grid is padded to a multiple of
+    // 8 and surplus blocks leave before any barrier.
+    uint64_t cg;
+    uint32_t rs;
+    if (a.xcd_affinity) {
+        const uint32_t xcd = blockIdx.x % 8, within = blockIdx.x / 8, G = 8 / a.row_slices;
+        rs = xcd % a.row_slices;
+        cg = (uint64_t)within * G + xcd / a.row_slices;
+        if (cg >= a.col_groups) return;
+    } else {
+        cg = blockIdx.x / a.row_slices;
+        rs = blockIdx.x % a.row_slices;
+    }
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int wr = wave / WC, wc = wave % WC;
+    const uint32_t wg_row0 = rs * ROWS;
+
+    GlobalBinom C{a.binom};
+    if (tid < SLOTS) {
+        const uint64_t u = cg * SLOTS + tid;
+        s_mask[tid] = u < a.u_out ? unrank_mask(u, (int)a.n, (int)a.deg_out, C) : 0ULL;
+    }
+    if (tid < kMaxN) s_any[tid] = 0;
+    __syncthreads();
+    for (int idx = tid; idx < SLOTS * (int)a.n; idx += NT) {
+        const int g = idx / (int)a.n, i = idx % (int)a.n;
+        const uint64_t u = cg * SLOTS + g;
+        uint32_t code = 0xffffffffu;
+        if (u < a.u_out) {
+            const uint64_t mask = s_mask[g];
+            const bool bit = (mask >> i) & 1ULL;
+            if (FWD ? !bit : bit) {
+                const uint32_t neg = (uint32_t)__popcll(mask & ((1ULL << i) - 1)) & 1u;
+                const uint64_t nb = FWD ? rank_mask(mask | (1ULL << i), (int)a.n, (int)a.deg_out + 1, C)
+                                        : rank_mask(mask & ~(1ULL << i), (int)a.n, (int)a.deg_out - 1, C);
+                code = (uint32_t)nb | (neg << 31);
+                s_any[i] = 1;  // benign race: every writer stores 1
+            }
+        }
+        s_code[g * kMaxN + i] = code;
+    }
+    __syncthreads();
+
+    v4i acc[RT][CT][NS];
+    static_for<RT>([&](auto rt) {
+        static_for<CT>([&](auto ct) { static_for<NS>([&](auto s) { acc[rt][ct][s] = v4i{0, 0, 0, 0}; }); });
+    });
+
+    const uint64_t plane = (uint64_t)NL * B * a.k_stride;  // bytes per panel and sign
+    const int n = (int)a.n;
+    const uint32_t k_pad = a.k_pad;
+
+    // stage = (i, k0): advance to the next table that some panel here is incident to
+    auto next_i = [&](int i) {
+        while (i < n && !s_any[i]) ++i;
+        return i;
+    };
+    // Per-thread constants of the staging copies, hoisted out of the stage loop:
+    // the 16-byte piece of the chunk row each item moves, its offset in the
+    // global table / digit plane (without the per-stage table or panel base) and
+    // its LDS offset.  Thread t moves items t + q*NT; piece = item % 4.
+    const int piece = tid % 4;
+    uint32_t a_goff[A_PER];
+    int a_soff[A_PER];
+    bool a_ok[A_PER];
+    static_for<A_PER>([&](auto q) {
+        const int item = tid + q * NT;
+        const int rest = item / 4;  // rest = j*ROWS + r
+        const int r = rest % ROWS, j = rest / ROWS;
+        a_ok[q] = item < A_ITEMS && wg_row0 + r < a.r_pad;
+        a_goff[q] = ((uint32_t)j * a.r_pad + wg_row0 + r) * k_pad + piece * 16;
+        a_soff[q] = lds_off(rest, piece * 16);
+    });
+    uint32_t b_goff[B_PER];
+    int b_soff[B_PER], b_code_idx[B_PER];
+    bool b_ok[B_PER];
+    static_for<B_PER>([&](auto q) {
+        const int item = tid + q * NT;
+        const int rest = item / 4;  // rest = (tile*NL + l)*16 + cc
+        const int c16 = rest % 16, rest2 = rest / 16;
+        const int l = rest2 % NL, tile = rest2 / NL;
+        b_ok[q] = item < B_ITEMS;
+        b_goff[q] = ((uint32_t)l * B + c16 % B) * a.k_stride + piece * 16;
+        b_soff[q] = lds_off(rest, piece * 16);
+        b_code_idx[q] = b_ok[q] ? (tile * SPT + c16 / B) * kMaxN : 0;
+    });
+    const uint64_t table_stride = (uint64_t)NL * a.r_pad * k_pad;  // bytes per table i
+
+    v4i ra[A_PER], rb[B_PER];
+    auto load_regs = [&](int i, uint32_t k0) {
+        const bool piece_ok = (k_pad - k0) >= 64 || piece < 2;  // tail chunks are 32 wide
+        const int8_t* abase = a.tabl + (uint64_t)i * table_stride + k0;
+        static_for<A_PER>([&](auto q) {
+            v4i v = v4i{0, 0, 0, 0};
+            if (a_ok[q] && piece_ok) v = *reinterpret_cast<const v4i*>(abase + a_goff[q]);
+            ra[q] = v;
+        });
+        static_for<B_PER>([&](auto q) {
+            v4i v = v4i{0, 0, 0, 0};
+            const uint32_t code = s_code[b_code_idx[q] + i];
+            if (b_ok[q] && piece_ok && code != 0xffffffffu)
+                v = *reinterpret_cast<const v4i*>(
+                    a.lp + ((uint64_t)(code >> 31) * a.v_in + (code & 0x7fffffffu)) * plane + k0 +
+                    b_goff[q]);
+            rb[q] = v;
+        });
+    };
+    auto store_lds = [&](int buf) {
+        static_for<A_PER>([&](auto q) {
+            if (tid + q * NT < A_ITEMS) *reinterpret_cast<v4i*>(&s_a[buf][a_soff[q]]) = ra[q];
+        });
+        static_for<B_PER>([&](auto q) {
+            if (b_ok[q]) *reinterpret_cast<v4i*>(&s_b[buf][b_soff[q]]) = rb[q];
+        });
+    };
+
+    const int cc = lane % 16;
+    const int my_slot = cc / B, my_c = cc % B;
+    // row tiles of this wave that hold table rows (wave-uniform)
+    bool row_on[RT];
+    static_for<RT>([&](auto rt) { row_on[rt] = wg_row0 + (wr * RT + rt) * 16 < a.r_pad; });
+
+    // Two phases, each a double-buffered pipeline over its stages: all 64-wide
+    // chunks of every incident table, then the 32-wide tail chunks.  Keeping one
+    // MFMA shape per loop keeps the accumulators in place across iterations.
+    const uint32_t k_full = k_pad / 64 * 64;
+    auto phase = [&](auto wide_c) {
+        constexpr bool WIDE = decltype(wide_c)::value;
+        const uint32_t kbeg = WIDE ? 0u : k_full, kend = WIDE ? k_full : k_pad;
+        if (kbeg >= kend) return;
+        int i = next_i(0);
+        uint32_t k0 = kbeg;
+        int buf = 0;
+        if (i < n) {
+            load_regs(i, k0);
+            store_lds(0);
+        }
+        __syncthreads();
+        while (i < n) {
+            int ni = i;
+            uint32_t nk = k0 + 64;
+            if (nk >= kend) {
+                ni = next_i(i + 1);
+                nk = kbeg;
+            }
+            const bool more = ni < n;
+            if (more) load_regs(ni, nk);  // in flight during the MFMAs below
+
+            bool on[CT];
+            static_for<CT>([&](auto ct) {
+                bool o = false;
+                static_for<SPT>([&](auto q) {
+                    o |= s_code[((wc * CT + ct) * SPT + q) * kMaxN + i] != 0xffffffffu;
+                });
+                on[ct] = __builtin_amdgcn_readfirstlane((int)o) != 0;  // uniform: scalar branch
+            });
+            const int8_t* sa = s_a[buf];
+            const int8_t* sb = s_b[buf];
+            if constexpr (WIDE) {
+                const int kq = (lane / 16) * 16;
+                v4i fa[RT][NL], fb[CT][NL];
+                static_for<RT>([&](auto rt) {
+                    static_for<NL>([&](auto j) {
+                        fa[rt][j] = *reinterpret_cast<const v4i*>(
+                            sa + (j * ROWS + wr * RT * 16 + rt * 16) * kLdsStride + lds_off(cc, kq));
+                    });
+                });
+                static_for<CT>([&](auto ct) {
+                    static_for<NL>([&](auto l) {
+                        fb[ct][l] = *reinterpret_cast<const v4i*>(
+                            sb + ((wc * CT + ct) * NL + l) * 16 * kLdsStride + lds_off(cc, kq));
+                    });
+                });
+                static_for<CT>([&](auto ct) {
+                    if (!on[ct]) return;
+                    static_for<RT>([&](auto rt) {
+                        if (!row_on[rt]) return;
+                        static_for<NL>([&](auto j) {
+                            static_for<NL>([&](auto l) {
+                                acc[rt][ct][j + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                    fa[rt][j], fb[ct][l], acc[rt][ct][j + l], 0, 0, 0);
+                            });
+                        });
+                    });
+                });
+            } else {
+                const int kq = (lane / 16) * 8;
+                long fa[RT][NL], fb[CT][NL];
+                static_for<RT>([&](auto rt) {
+                    static_for<NL>([&](auto j) {
+                        fa[rt][j] = *reinterpret_cast<const long*>(
+                            sa + (j * ROWS + wr * RT * 16 + rt * 16) * kLdsStride + lds_off(cc, kq));
+                    });
+                });
+                static_for<CT>([&](auto ct) {
+                    static_for<NL>([&](auto l) {
+                        fb[ct][l] = *reinterpret_cast<const long*>(
+                            sb + ((wc * CT + ct) * NL + l) * 16 * kLdsStride + lds_off(cc, kq));
+                    });
+                });
+                static_for<CT>([&](auto ct) {
+                    if (!on[ct]) return;
+                    static_for<RT>([&](auto rt) {
+                        if (!row_on[rt]) return;
+                        static_for<NL>([&](auto j) {
+                            static_for<NL>([&](auto l) {
+                                acc[rt][ct][j + l] = __builtin_amdgcn_mfma_i32_16x16x32_i8(
+                                    fa[rt][j], fb[ct][l], acc[rt][ct][j + l], 0, 0, 0);
+                            });
+                        });
+                    });
+                });
+            }
+            if (more) store_lds(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+            i = ni;
+            k0 = nk;
+        }
+    };
+    phase(std::true_type{});
+    phase(std::false_type{});
+
+    // epilogue: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile
+    const uint64_t off = (uint64_t)a.m.p << 32;  // multiple of p above 2^31
+    static_for<RT>([&](auto rt) {
+        static_for<CT>([&](auto ct) {
+            const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
+            static_for<4>([&](auto v) {
+                const uint32_t row = wg_row0 + (wr * RT + rt) * 16 + (lane / 16) * 4 + v;
+                if (u >= a.u_out || row >= a.r_out) return;
+                uint64_t s_lo = 0, s_hi = 0;
+                static_for<NS>([&](auto s) {
+                    const int32_t a32 = acc[rt][ct][s][decltype(v)::value];
+                    const uint32_t r = mod_u64((uint64_t)((int64_t)a32 + (int64_t)off), a.m);
+                    const uint64_t prod = (uint64_t)r * a.sw.w[s];  // < 2^62
+                    if (s < 4) s_lo += prod; else s_hi += prod;     // 4 terms < 2^64
+                });
+                uint32_t val = addmod(mod_u64(s_lo, a.m), mod_u64(s_hi, a.m), a.m.p);
+                const uint64_t orow = u * a.r_out + row;
+                if (a.scale) val = mulmod(val, a.scale[orow], a.m);
+                a.out[orow * B + my_c] = val;
+            });
+        });
+    });
+}
+
+template <int B, int NL, int RT, int CT, int WR, int WC>
+static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
+                           const uint32_t* scale, const ModP& m, hipStream_t st) {
+    constexpr int SLOTS = WC * CT * (16 / B);
+    constexpr int ROWS = WR * RT * 16;
+    require(s.n <= (uint32_t)kMaxN, "MFMA sweep supports n <= 32");
+    MfmaSweepArgs a;
+    a.binom = s.binom;
+    require(a.binom != nullptr, "MFMA sweep needs the device binomial table");
+    a.tabl = t.d;
+    a.lp = lp;
+    a.n = s.n;
+    a.deg_out = s.fwd ? s.k - 1 : s.k;
+    a.u_out = s.u_out;
+    a.v_in = s.v_in;
+    a.r_out = s.r_out;
+    a.r_pad = t.rows_pad;
+    a.k_in = s.k_in;
+    a.k_pad = t.k_pad;
+    a.k_stride = limb_stride(s.k_in);
+    a.row_slices = (t.rows_pad + ROWS - 1) / ROWS;
+    a.col_groups = (s.u_out + SLOTS - 1) / SLOTS;
+    a.out = out;
+    a.scale = scale;
+    a.m = m;
+    uint64_t w = 1;
+    for (int q = 0; q < 8; ++q) {
+        a.sw.w[q] = (uint32_t)w;
+        w = (w * 256) % m.p;
+    }
+    a.xcd_affinity = (8 % a.row_slices == 0) ? 1u : 0u;
+    const uint64_t G = 8 / a.row_slices;
+    const uint64_t blocks = a.xcd_affinity ? 8 * ((a.col_groups + G - 1) / G)
+                                           : a.col_groups * a.row_slices;
+    require(blocks < (1ULL << 31), "MFMA sweep grid too large");
+#define PGW_MF(F)                                                                                 \
+    hipLaunchKernelGGL((k_dense_mfma<B, NL, RT, CT, WR, WC, F>), dim3((unsigned)blocks),          \
+                       dim3(64 * WR * WC), 0, st, a)
+    if (s.fwd) PGW_MF(true); else PGW_MF(false);
+#undef PGW_MF
+    PGW_CHECK_LAUNCH("MFMA sweep");
+}
+
+template <int B, int NL>
+static void dense_mfma_pick(int cfg, const DenseSweep& s, const MfmaTable& t, const int8_t* lp,
+                            uint32_t* out, const uint32_t* scale, const ModP& m, hipStream_t st) {
+    if constexpr (NL == 4) {
+        switch (cfg) {
+            case 0: dense_mfma_cfg<B, NL, 2, 2, 2, 2>(s, t, lp, out, scale, m, st); return;
+            case 1: dense_mfma_cfg<B, NL, 2, 2, 4, 2>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 2, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
+        }
+    } else {
+        switch (cfg) {
+            case 0: dense_mfma_cfg<B, NL, 2, 4, 2, 2>(s, t, lp, out, scale, m, st); return;
+            case 1: dense_mfma_cfg<B, NL, 2, 4, 4, 2>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 4, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
+        }
+    }
+}
+
+}  // namespace pgw

@@ -146,6 +146,28 @@ def test_small_synthetic_plain_and_mfma_vs_oracle(torch_cuda, ctx, p, b):
         assert (host(d_out) == o.apply_forward(v, b)).all(), method
 
 
+@pytest.mark.parametrize("cfg", [0, 1, 2])
+@pytest.mark.parametrize("p,b,n,k,M,Mp", [(P31, 16, 9, 4, 37, 45), (65521, 8, 10, 3, 70, 29),
+                                          (P31, 4, 7, 2, 64, 100)])
+def test_mfma_tile_configs_vs_oracle(torch_cuda, ctx, monkeypatch, cfg, p, b, n, k, M, Mp):
+    # every tile configuration the autotuner may choose, on ragged panel sizes
+    spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=21)
+    o = oc.synthetic_op(n, k, M, Mp, p, T, E, dr, dc, 2, pairs.reshape(-1))
+    monkeypatch.setenv("PGW_MFMA_CFG", str(cfg))
+    op = pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=pg.METHOD_MFMA, tables=(T, E))
+    assert (gpu_apply(torch_cuda, op, v) == o.apply_square(v, b)).all()
+
+
+def test_autotune_keeps_results(torch_cuda, ctx):
+    n, k, M, Mp, p, b = 9, 4, 40, 44, P31, 16
+    spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=2)
+    op = pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=pg.METHOD_MFMA, tables=(T, E))
+    before = gpu_apply(torch_cuda, op, v)
+    choice, ms = op.autotune(2)
+    assert len(choice) == 4 and all(0 <= c < 3 for c in choice) and all(t > 0 for t in ms)
+    assert (gpu_apply(torch_cuda, op, v) == before).all()
+
+
 def test_odd_rows_and_plain_only_block_widths(torch_cuda, ctx):
     # N odd (one row left out of every round's pairs), b = 3 runs on the plain path
     n, k, M, Mp, p, b = 7, 3, 3, 4, 1000003, 3  # N = 35 * 3 = 105
