@@ -81,6 +81,10 @@ EXPORTED = [
     "pgw_synthetic_table", "pgw_operator_dims", "pgw_operator_create", "pgw_operator_destroy",
     "pgw_operator_info", "pgw_apply_square", "pgw_apply_forward", "pgw_gram_upper",
     "pgw_sequence_run", "pgw_kernel_vectors", "pgw_bench_apply", "pgw_operator_autotune",
+    "pgw_batch_create", "pgw_batch_add", "pgw_batch_step", "pgw_batch_poll", "pgw_batch_result",
+    "pgw_batch_cancel", "pgw_batch_graph_builds", "pgw_batch_destroy", "pgw_server_create",
+    "pgw_server_submit", "pgw_server_wait", "pgw_server_result", "pgw_server_stats",
+    "pgw_server_destroy", "pgw_job_footprint",
 ]
 
 
@@ -128,6 +132,23 @@ def lib():
         "pgw_bench_apply": (I, [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u32)]),
         "pgw_operator_autotune": (I, [vp, u32, vp, vp]),
+        "pgw_batch_create": (I, [vp, u32, I, ctypes.POINTER(vp)]),
+        "pgw_batch_add": (I, [vp, vp, vp, u64, ctypes.POINTER(u64)]),
+        "pgw_batch_step": (I, [vp, u32, ctypes.POINTER(u64)]),
+        "pgw_batch_poll": (I, [vp, u64, ctypes.POINTER(I)]),
+        "pgw_batch_result": (I, [vp, u64, vp, vp, ctypes.POINTER(u64)]),
+        "pgw_batch_cancel": (I, [vp, u64]),
+        "pgw_batch_graph_builds": (I, [vp, ctypes.POINTER(u64)]),
+        "pgw_batch_destroy": (I, [vp]),
+        "pgw_server_create": (I, [vp, u64, u32, u32, ctypes.POINTER(vp)]),
+        "pgw_server_submit": (I, [vp, ctypes.POINTER(OperatorDesc), vp, vp, u32, vp, vp, u64,
+                                  ctypes.POINTER(u64)]),
+        "pgw_server_wait": (I, [vp, u64, ctypes.c_double, ctypes.POINTER(I)]),
+        "pgw_server_result": (I, [vp, u64, vp, vp, ctypes.POINTER(u64), vp]),
+        "pgw_server_stats": (I, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                 ctypes.POINTER(u64)]),
+        "pgw_server_destroy": (I, [vp]),
+        "pgw_job_footprint": (I, [ctypes.POINTER(OperatorDesc), u64, u32, ctypes.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -390,6 +411,138 @@ class Operator:
     def close(self):
         if getattr(self, "h", None):
             lib().pgw_operator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many sector sequence loops advanced together (pgw_batch_*)."""
+
+    def __init__(self, ctx: Context, nstreams: int = 8, use_graph: bool = True):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().pgw_batch_create(ctx.h, nstreams, 1 if use_graph else 0, ctypes.byref(h)))
+        self.h = h
+        self.jobs = {}
+
+    def add(self, op: Operator, v, max_len: int) -> int:
+        v = _u32(v).reshape(-1)
+        job = ctypes.c_uint64()
+        _check(lib().pgw_batch_add(self.h, op.h, _ptr(v), max_len, ctypes.byref(job)))
+        self.jobs[job.value] = (op, max_len)
+        return job.value
+
+    def step(self, max_rounds: int = 1) -> int:
+        active = ctypes.c_uint64()
+        _check(lib().pgw_batch_step(self.h, max_rounds, ctypes.byref(active)))
+        return active.value
+
+    def done(self, job: int) -> bool:
+        d = ctypes.c_int()
+        _check(lib().pgw_batch_poll(self.h, job, ctypes.byref(d)))
+        return bool(d.value)
+
+    def result(self, job: int):
+        op, max_len = self.jobs.pop(job)
+        b = op.block
+        ut = b * (b + 1) // 2
+        L = sequence_length(max_len)
+        seq = np.empty(L * ut, np.uint32)
+        curv = np.empty(op.cols * b, np.uint32)
+        n = ctypes.c_uint64()
+        _check(lib().pgw_batch_result(self.h, job, _ptr(seq), _ptr(curv), ctypes.byref(n)))
+        assert n.value == L
+        return seq.reshape(L, ut), curv
+
+    def run(self, max_rounds: int = 0):
+        """Step until every job is done; returns {job: (seq, curv)}."""
+        while self.step(max_rounds or (1 << 30)):
+            pass
+        return {j: self.result(j) for j in list(self.jobs)}
+
+    def graph_builds(self) -> int:
+        n = ctypes.c_uint64()
+        _check(lib().pgw_batch_graph_builds(self.h, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_footprint(desc: OperatorDesc, max_len: int, rounds: int = 2) -> int:
+    """Device bytes a sector job is admitted with (pgw_job_footprint)."""
+    out = ctypes.c_uint64()
+    _check(lib().pgw_job_footprint(ctypes.byref(desc), max_len, rounds, ctypes.byref(out)))
+    return out.value
+
+
+class Server:
+    """Sector-job serving loop with HBM-footprint admission (pgw_server_*)."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_active: int = 16,
+                 rounds_per_step: int = 8):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().pgw_server_create(ctx.h, hbm_budget, max_active, rounds_per_step,
+                                       ctypes.byref(h)))
+        self.h = h
+        self.meta = {}
+
+    def submit(self, desc: OperatorDesc, dr, dc, v, max_len: int, rounds: int = 0,
+               pairs=None, keep=None) -> int:
+        """Queue a job; desc's arrays (kept alive by `keep`) are copied at once."""
+        dr, dc, v = _u32(dr), _u32(dc), _u32(v).reshape(-1)
+        pairs_arr = _u32(pairs).reshape(-1, 4) if pairs is not None and len(pairs) else None
+        t = ctypes.c_uint64()
+        _check(lib().pgw_server_submit(self.h, ctypes.byref(desc), _ptr(dr), _ptr(dc), rounds,
+                                       _ptr(pairs_arr) if pairs_arr is not None else None,
+                                       _ptr(v), max_len, ctypes.byref(t)))
+        del keep
+        self.meta[t.value] = (desc.block, dc.size, max_len)
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> int:
+        st = ctypes.c_int()
+        _check(lib().pgw_server_wait(self.h, ticket, timeout, ctypes.byref(st)))
+        return st.value
+
+    def result(self, ticket: int):
+        """(seq, curv, (t_submit, t_admit, t_finish)) of a finished job."""
+        b, cols, max_len = self.meta.pop(ticket)
+        ut = b * (b + 1) // 2
+        L = sequence_length(max_len)
+        seq = np.empty(L * ut, np.uint32)
+        curv = np.empty(cols * b, np.uint32)
+        times = np.zeros(3, np.float64)
+        n = ctypes.c_uint64()
+        _check(lib().pgw_server_result(self.h, ticket, _ptr(seq), _ptr(curv), ctypes.byref(n),
+                                       _ptr(times)))
+        return seq.reshape(L, ut), curv, tuple(times.tolist())
+
+    def stats(self):
+        c, pb, pa, q = (ctypes.c_uint64() for _ in range(4))
+        _check(lib().pgw_server_stats(self.h, ctypes.byref(c), ctypes.byref(pb), ctypes.byref(pa),
+                                      ctypes.byref(q)))
+        return dict(completed=c.value, peak_bytes=pb.value, peak_active=pa.value, queued=q.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgw_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -55,9 +55,12 @@ void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64
 // Gram projection -----------------------------------------------------------
 // mode 0: out0 = X^T Y; mode 1: out0 = X^T Y, out1 = Y^T Y; mode 2: out0 = X^T Y, out1 = X^T X.
 size_t gram_scratch_bytes(uint32_t b);
+// pos (optional, device): outputs land ut * (*pos + pos_off) entries after out0/out1
 void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
                  uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
-                 hipStream_t st);
+                 hipStream_t st, const uint32_t* pos = nullptr, uint32_t pos_off = 0);
+// *pos += by (one thread)
+void launch_advance_pos(uint32_t* pos, uint32_t by, hipStream_t st);
 
 // kernel-vector generator slice: D[r][c] += sum_i B[r][i] G[c*b+i]
 void launch_accumulate_generator(const uint32_t* B, const uint32_t* G, uint32_t* D, uint64_t rows,

@@ -578,13 +578,23 @@ __global__ __launch_bounds__(256) void k_gram_partial(const uint32_t* __restrict
     }
 }
 
+// With `pos` set, out0/out1 are offset by ut * (*pos + pos_off) entries: the
+// sequence position is read on the device, so a captured graph of a whole
+// round can be replayed unchanged.
 __global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict__ part,
                                                      unsigned nblocks, uint32_t b, int mode,
                                                      uint32_t* __restrict__ out0,
-                                                     uint32_t* __restrict__ out1, ModP m) {
+                                                     uint32_t* __restrict__ out1, ModP m,
+                                                     const uint32_t* __restrict__ pos,
+                                                     uint32_t pos_off) {
     const int t = threadIdx.x;
     const int i = t / 16, j = t % 16;
     if (i >= (int)b || j >= (int)b || i > j) return;
+    if (pos) {
+        const uint64_t shift = (uint64_t)(b * (b + 1) / 2) * (*pos + pos_off);
+        out0 += shift;
+        if (out1) out1 += shift;
+    }
     const int ut = i * (2 * (int)b - i + 1) / 2 + (j - i);
     for (int mat = 0; mat < (mode ? 2 : 1); ++mat) {
         uint64_t lo = 0, hi = 0;
@@ -597,9 +607,18 @@ __global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict_
     }
 }
 
+__global__ void k_advance_pos(uint32_t* pos, uint32_t by) {
+    if (threadIdx.x == 0) *pos += by;
+}
+
+void launch_advance_pos(uint32_t* pos, uint32_t by, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance_pos, dim3(1), dim3(64), 0, st, pos, by);
+    PGW_CHECK_LAUNCH("advance position");
+}
+
 void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
                  uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
-                 hipStream_t st) {
+                 hipStream_t st, const uint32_t* pos, uint32_t pos_off) {
     // low words: rows * 2^32 < 2^64 requires rows < 2^32
     require(rows < (1ULL << 32), "Gram projection supports fewer than 2^32 rows");
     unsigned grid = (unsigned)std::min<uint64_t>(kGramGrid, std::max<uint64_t>(1, (rows + kGramRows - 1) / kGramRows));
@@ -608,7 +627,7 @@ void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b
     PGW_SWITCH_B(b, PGW_CALL)
 #undef PGW_CALL
     hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(256), 0, st, scratch, grid, b, mode, out0,
-                       out1, m);
+                       out1, m, pos, pos_off);
     PGW_CHECK_LAUNCH("gram");
 }
 

@@ -1,0 +1,178 @@
+"""GPU: batches of uneven sectors and the serving loop, through the C ABI.
+
+Every job's sequence and final block must be bit-identical to the same
+operator's lone pgw_sequence_run, and (for the small sectors) to the CPU
+oracle's sequence loop; results must not depend on the mix they ran in.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as oc
+import prym_green_ancillary_amd as pg
+from prym_green_ancillary_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    c = pg.Context(0)
+    yield c
+    c.close()
+
+
+def small_mix(count=6, seed=3):
+    # the uneven-sector recipe at oracle-sized shapes
+    return wl.uneven_sectors(seed=seed, count=count, n_lo=7, n_hi=11, k_lo=2, k_hi=4, m_lo=4,
+                             m_hi=24, n_min=50, n_max=12000)
+
+
+def make_op(ctx, job, method, seed=3):
+    dr, dc, v, pairs = wl.sector_state(job, seed)
+    T, E = job.spec.tables()
+    op = pg.Operator.synthetic(ctx, job.spec, job.block, dr, dc, 2, pairs, method=method,
+                               tables=(T, E))
+    return op, (T, E, dr, dc, v, pairs)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_uneven_sectors_match_single_runs_and_oracle(ctx, use_graph):
+    jobs = small_mix()
+    lengths = [5 + 7 * i for i in range(len(jobs))]  # uneven lengths: jobs finish at different steps
+    batch = pg.Batch(ctx, nstreams=4, use_graph=use_graph)
+    ops, ids, state = [], [], []
+    for j, L in zip(jobs, lengths):
+        method = pg.METHOD_MFMA if j.index % 2 == 0 else pg.METHOD_PLAIN
+        op, st = make_op(ctx, j, method)
+        _log(f"batch add sector {j.index}: n={j.spec.n} k={j.spec.k} M={j.spec.m_src} "
+             f"b={j.block} N={j.cols} len={L} method={method}")
+        ids.append(batch.add(op, st[4], L))
+        ops.append(op)
+        state.append(st)
+    steps = 0
+    while batch.step(3):
+        steps += 1
+    got = {i: batch.result(i) for i in ids}
+    if use_graph:
+        assert batch.graph_builds() >= 2  # the active set shrank at least once
+    batch.close()
+    for j, L, op, st, i in zip(jobs, lengths, ops, state, ids):
+        T, E, dr, dc, v, pairs = st
+        seq, curv = got[i]
+        lone_seq, lone_curv = op.sequence(v, L)
+        assert (seq == lone_seq).all() and (curv == lone_curv).all(), j.index
+        o = oc.synthetic_op(j.spec.n, j.spec.k, j.spec.m_src, j.spec.m_tgt, j.spec.prime, T, E,
+                            dr, dc, 2, pairs.reshape(-1))
+        oseq, ocurv = o.sequence(v, j.block, L)
+        assert (seq == oseq).all() and (curv == ocurv).all(), j.index
+
+
+def test_batch_mixes_cyclic_and_synthetic(ctx, g12):
+    p, b = g12["modulus"], 4
+    batch = pg.Batch(ctx, nstreams=3)
+    entries = []
+    for sector in range(g12["cyclic"]["order"]):
+        o = oc.cyclic_op(g12, sector)
+        dr, dc, v = pg.reference_random_state(1, sector, p, o.rows, o.cols, b)
+        realized = pg.rowmix_realized_seed(10001, "eliminated", sector, o.rows, p)
+        pairs = pg.rowmix_plan(realized, o.rows, 2, p)
+        o.set_preconditioners(dr, dc, 2, pairs.reshape(-1))
+        op = pg.Operator.cyclic(ctx, g12, sector, b, dr, dc, 2, pairs)
+        L = 2 * o.cols // b + 50
+        entries.append((batch.add(op, v, L), op, o, v, L, b))
+    j = small_mix(count=1, seed=11)[0]
+    op, st = make_op(ctx, j, pg.METHOD_MFMA, seed=11)
+    entries.append((batch.add(op, st[4], 40), op, None, st[4], 40, j.block))
+    res = batch.run()
+    batch.close()
+    for jid, op, o, v, L, bb in entries:
+        seq, curv = res[jid]
+        lone = op.sequence(v, L)
+        assert (seq == lone[0]).all() and (curv == lone[1]).all()
+        if o is not None:
+            oseq, ocurv = o.sequence(v, bb, L)
+            assert (seq == oseq).all() and (curv == ocurv).all()
+
+
+def test_batch_rejects_second_job_on_one_operator(ctx):
+    j = small_mix(count=1)[0]
+    op, st = make_op(ctx, j, pg.METHOD_PLAIN)
+    batch = pg.Batch(ctx)
+    batch.add(op, st[4], 9)
+    with pytest.raises(pg.PgwError, match="one active job"):
+        batch.add(op, st[4], 9)
+    batch.run()
+    batch.close()
+
+
+def test_server_results_identical_alone_and_in_mix(ctx):
+    jobs = small_mix(count=8, seed=7)
+    lengths = [9 + 5 * (i % 4) for i in range(len(jobs))]
+    setups = []
+    for j in jobs:
+        dr, dc, v, pairs = wl.sector_state(j, 7)
+        T, E = j.spec.tables()
+        desc = pg.Operator.synthetic_desc(j.spec, j.block, pg.METHOD_AUTO, T, E)
+        setups.append((desc, (T, E), dr, dc, v, pairs))
+    foot = [pg.job_footprint(s[0], L, 2) for s, L in zip(setups, lengths)]
+    budget = int(sum(sorted(foot)[-3:]))  # room for about three jobs at once
+    srv = pg.Server(ctx, budget, max_active=4, rounds_per_step=2)
+    tickets = []
+    for s, L in zip(setups, lengths):
+        desc, keep, dr, dc, v, pairs = s
+        tickets.append(srv.submit(desc, dr, dc, v, L, 2, pairs, keep=keep))
+    outs = []
+    for t in tickets:
+        assert srv.wait(t, 120.0) == pg.Server.DONE
+        outs.append(srv.result(t))
+    stats = srv.stats()
+    srv.close()
+    _log(f"server stats {stats} budget {budget}")
+    assert stats["completed"] == len(jobs)
+    assert stats["peak_bytes"] <= budget
+    assert 1 <= stats["peak_active"] <= 4
+    for j, s, L, (seq, curv, times) in zip(jobs, setups, lengths, outs):
+        desc, (T, E), dr, dc, v, pairs = s
+        op = pg.Operator.synthetic(ctx, j.spec, j.block, dr, dc, 2, pairs, tables=(T, E))
+        lone_seq, lone_curv = op.sequence(v, L)
+        assert (seq == lone_seq).all() and (curv == lone_curv).all(), j.index
+        assert times[0] <= times[1] <= times[2]
+
+
+def test_server_refuses_job_larger_than_budget(ctx):
+    j = small_mix(count=1)[0]
+    dr, dc, v, pairs = wl.sector_state(j, 3)
+    T, E = j.spec.tables()
+    desc = pg.Operator.synthetic_desc(j.spec, j.block, pg.METHOD_AUTO, T, E)
+    srv = pg.Server(ctx, 1 << 20, max_active=2)
+    t = srv.submit(desc, dr, dc, v, 9, 2, pairs, keep=(T, E))
+    assert srv.wait(t, 60.0) == pg.Server.FAILED
+    with pytest.raises(pg.PgwError, match="more device memory"):
+        srv.result(t)
+    srv.close()
+
+
+def test_footprint_bounds_what_an_operator_holds(ctx):
+    for j in small_mix(count=4, seed=9) + wl.uneven_sectors(count=2):
+        dr, dc, v, pairs = wl.sector_state(j, 9)
+        T, E = j.spec.tables()
+        for method in (pg.METHOD_MFMA, pg.METHOD_PLAIN):
+            desc = pg.Operator.synthetic_desc(j.spec, j.block, method, T, E)
+            L = 33
+            est = pg.job_footprint(desc, L, 2)
+            op = pg.Operator.synthetic(ctx, j.spec, j.block, dr, dc, 2, pairs, method=method,
+                                       tables=(T, E))
+            b = j.block
+            actual = op.device_bytes + 2 * j.cols * b * 4 + pg.sequence_length(L) * b * (b + 1) // 2 * 4
+            assert actual <= est, (j.index, method, actual, est)
+            op.close()
