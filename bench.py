@@ -69,6 +69,172 @@ def cpu_baseline(spec, T, E, dr, dc, pairs, v, b, budget_s=20.0):
     }
 
 
+def sector_macs(spec, b):
+    """GF(p) multiply-adds of one block-apply (four sweeps of equal work)."""
+    return 4 * macs_per_sweep(spec.n, spec.k, spec.m_src, spec.m_tgt, b)
+
+
+def bench_batch(args):
+    """64 uneven sectors (seed 3): `--rounds` sequence rounds of every sector,
+    all in one batch (graph-replayed rounds over 8 streams) against the same
+    sectors run one at a time.  Results of the two runs must be identical."""
+    import prym_green_ancillary_amd as pg
+    from prym_green_ancillary_amd import workloads as wl
+
+    jobs = wl.uneven_sectors(seed=3, count=args.sectors)
+    ctx = pg.Context(0)
+    dev_name, _ = ctx.device_info()
+    t_setup = time.perf_counter()
+    ops, vs = [], []
+    for j in jobs:
+        dr, dc, v, pairs = wl.sector_state(j, 3)
+        T, E = j.spec.tables()
+        ops.append(pg.Operator.synthetic(ctx, j.spec, j.block, dr, dc, 2, pairs, tables=(T, E)))
+        vs.append(v)
+    t_setup = time.perf_counter() - t_setup
+    max_len = 1 + 4 * args.rounds
+    applies = 2 * args.rounds * len(jobs)
+    macs = sum(2 * args.rounds * sector_macs(j.spec, j.block) for j in jobs)
+
+    def run(groups, nstreams, balance=False):
+        out, secs = {}, 0.0
+        for g in groups:
+            batch = pg.Batch(ctx, nstreams=nstreams, use_graph=True, balance=balance)
+            ids = [(i, batch.add(ops[i], vs[i], max_len)) for i in g]
+            t0 = time.perf_counter()
+            while batch.step(1 << 20):
+                pass
+            secs += time.perf_counter() - t0
+            for i, jid in ids:
+                out[i] = batch.result(jid)
+            batch.close()
+        return out, secs
+
+    # warm the code paths once (untimed)
+    run([[0]], 1)
+    together, t_batch = run([list(range(len(jobs)))], 8)
+    balanced, t_bal = run([list(range(len(jobs)))], 8, balance=True)
+    alone, t_alone = run([[i] for i in range(len(jobs))], 1)
+    for i in range(len(jobs)):
+        for other in (together, balanced):
+            if not ((other[i][0] == alone[i][0]).all() and (other[i][1] == alone[i][1]).all()):
+                print(f"sector {i}: batched and lone results differ: no number reported",
+                      file=sys.stderr)
+                sys.exit(1)
+    Ns = [j.cols for j in jobs]
+    line = {
+        "metric": "GF(p) block-applies/s over a batch of uneven sectors, bit-exact vs lone runs",
+        "value": round(applies / t_batch, 3),
+        "unit": "block-applies/s",
+        "n_gpus": 1, "steps": args.rounds, "warmup": 1,
+        "ms_per_step": round(1000.0 * t_batch / args.rounds, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8-limb/int32 (u32 residues mod 65521)", "data": "synthetic",
+        "config": {"workload": f"{len(jobs)} uneven sectors, seed 3: n 12..22, k 3..6, M 20..300, "
+                               f"M'=M+ceil(M/10), p=65521, b in {{8,16}}, N {min(Ns)}..{max(Ns)}",
+                   "sectors": len(jobs), "rounds_per_sector": args.rounds, "sum_N": sum(Ns),
+                   "parallelism": "single GPU, 8 streams, one HIP graph per round"},
+        "gf_macs_per_s": float(f"{macs / t_batch:.4g}"),
+        "one_at_a_time_block_applies_per_s": round(applies / t_alone, 3),
+        "balanced_quota_block_applies_per_s": round(applies / t_bal, 3),
+        "batch_vs_one_at_a_time": round(t_alone / t_batch, 3),
+        "bit_exact_batched_vs_lone": True,
+        "setup_s": round(t_setup, 2),
+        "device": dev_name,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Poisson stream (seed 5, 2 jobs/s) replayed in real time into the
+    serving loop; each job runs min(2N/b + 8, --max-len) sequence terms.
+    Reports submit-to-finish latency p50/p99 and throughput, against the same
+    stream through a server that runs one job at a time."""
+    import prym_green_ancillary_amd as pg
+    from prym_green_ancillary_amd import workloads as wl
+
+    jobs = wl.poisson_stream(seed=5, count=args.jobs, rate=2.0)
+    ctx = pg.Context(0)
+    dev_name, total = ctx.device_info()
+    prepared = []
+    t_setup = time.perf_counter()
+    for j in jobs:
+        dr, dc, v, pairs = wl.sector_state(j, 5)
+        T, E = j.spec.tables()
+        desc = pg.Operator.synthetic_desc(j.spec, j.block, pg.METHOD_AUTO, T, E)
+        L = min(j.full_length(), args.max_len)
+        prepared.append((j, desc, (T, E), dr, dc, v, pairs, L))
+    t_setup = time.perf_counter() - t_setup
+    budget = int(args.hbm_fraction * total)
+
+    def replay(max_active):
+        srv = pg.Server(ctx, budget, max_active=max_active, replays_per_step=1)
+        t0 = time.perf_counter()
+        tickets = []
+        for j, desc, keep, dr, dc, v, pairs, L in prepared:
+            delay = j.arrival - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(desc, dr, dc, v, L, 2, pairs, keep=keep))
+        res = []
+        for t in tickets:
+            if srv.wait(t, 1200.0) != pg.Server.DONE:
+                print("a serving job failed: no number reported", file=sys.stderr)
+                sys.exit(1)
+            res.append(srv.result(t))
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+        srv.close()
+        lat = [r[2][4] - r[2][0] for r in res]
+        return res, wall, lat, stats
+
+    def phases(res):
+        t = np.array([r[2] for r in res])
+        d = np.diff(t, axis=1).mean(axis=0)
+        return {"queued": round(float(d[0]), 3), "setup": round(float(d[1]), 3),
+                "in_batch": round(float(d[2]), 3), "results": round(float(d[3]), 3)}
+
+    mixed, wall, lat, stats = replay(args.max_active)
+    alone, wall1, lat1, _ = replay(1)
+    for i, (a, b1) in enumerate(zip(mixed, alone)):
+        if not ((a[0] == b1[0]).all() and (a[1] == b1[1]).all()):
+            print(f"job {i}: results differ between the mix and one-at-a-time", file=sys.stderr)
+            sys.exit(1)
+    applies = sum(2 * ((L - 1 + 3) // 4) for (_, _, _, _, _, _, _, L) in prepared)
+    Ns = [j.cols for j in jobs]
+    line = {
+        "metric": "sector job latency p50/p99 + throughput under a Poisson stream; results "
+                  "identical alone vs in the mix",
+        "value": round(float(np.percentile(lat, 99)), 3),
+        "unit": "s (p99 submit-to-finish)",
+        "n_gpus": 1, "steps": len(jobs), "warmup": 0,
+        "ms_per_step": round(1000.0 * wall / len(jobs), 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8-limb/int32 (u32 residues mod 65521 or 2^31-1)", "data": "synthetic",
+        "config": {"workload": f"Poisson stream seed 5, 2 jobs/s, {len(jobs)} jobs, N log-uniform "
+                               f"1e4..3e7 ({min(Ns)}..{max(Ns)}), p in {{65521, 2^31-1}}, "
+                               f"b in {{8,16}}, terms min(2N/b+8, {args.max_len})",
+                   "jobs": len(jobs), "hbm_budget_bytes": budget, "max_active": args.max_active,
+                   "parallelism": "single GPU, continuous batching"},
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
+                      "p99": round(float(np.percentile(lat, 99)), 3)},
+        "jobs_per_s": round(len(jobs) / wall, 3),
+        "block_applies_per_s": round(applies / wall, 3),
+        "mean_phase_s": phases(mixed),
+        "one_at_a_time": {"mean_phase_s": phases(alone),
+                          "latency_p50_s": round(float(np.percentile(lat1, 50)), 3),
+                          "latency_p99_s": round(float(np.percentile(lat1, 99)), 3),
+                          "jobs_per_s": round(len(jobs) / wall1, 3)},
+        "peak_active": stats["peak_active"], "peak_bytes": stats["peak_bytes"],
+        "bit_exact_mix_vs_alone": True,
+        "setup_s": round(t_setup, 2),
+        "device": dev_name,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,11 +243,24 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--workload", choices=["sector", "batch", "serve"], default="sector",
+                    help="sector: one mid-size sector (default); batch: 64 uneven sectors; "
+                         "serve: Poisson job stream")
+    ap.add_argument("--rounds", type=int, default=4, help="batch: sequence rounds per sector")
+    ap.add_argument("--sectors", type=int, default=64, help="batch: sectors in the mix")
+    ap.add_argument("--jobs", type=int, default=24, help="serve: jobs in the stream")
+    ap.add_argument("--max-len", type=int, default=33, help="serve: sequence-term cap per job")
+    ap.add_argument("--max-active", type=int, default=16, help="serve: concurrent jobs")
+    ap.add_argument("--hbm-fraction", type=float, default=0.5, help="serve: HBM budget share")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); the sector path "
               "has no multi-GPU mode in this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import torch
     import prym_green_ancillary_amd as pg

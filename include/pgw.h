@@ -208,26 +208,36 @@ int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters,
  * The reference runs one sector per process, one after another
  * (scripts/reproduce.py:235-276 launching cuda/cuprym_cyclic.cu:2316-2461
  * per sector).  A batch holds one job per operator (each job is exactly the
- * pgw_sequence_run loop of that operator) and advances every unfinished job
- * by whole rounds.  One round of all active jobs is captured once into a HIP
- * graph that fans the jobs out over `nstreams` streams; the graph is
- * replayed while the active set is unchanged and recaptured when it
- * changes.  Sequence positions live in a device counter per job, so the
- * same graph serves every round.  Each job's sequence and final block are
- * identical to pgw_sequence_run on the same operator and V.
+ * pgw_sequence_run loop of that operator) and advances the unfinished jobs
+ * by whole rounds.  One replay advances every job by its quota of rounds:
+ * 1, or with PGW_BATCH_BALANCE about (slowest job's round time / this job's
+ * round time), at most 8, from a cost model of the operator, so that small
+ * sectors do not wait on large ones.  With PGW_BATCH_GRAPH one replay is
+ * captured once into a HIP graph that fans the jobs out over `nstreams`
+ * streams, replayed while the jobs and quotas are unchanged and recaptured
+ * when they change.  Sequence positions live in a device counter per job,
+ * so the same graph serves every round.  Each job's sequence and final
+ * block are identical to pgw_sequence_run on the same operator and V.
+ * pgw_batch_add / _poll / _result may be called from another thread than
+ * pgw_batch_step (jobs are staged on a stream of their own).
  * ---------------------------------------------------------------------- */
 typedef struct pgw_batch pgw_batch;
 
-/* nstreams = 0 picks 8; use_graph = 0 launches every round eagerly. */
-int pgw_batch_create(pgw_ctx* ctx, uint32_t nstreams, int use_graph, pgw_batch** out);
+enum pgw_batch_flags {
+    PGW_BATCH_GRAPH = 1,    /* replay captured HIP graphs (else launch every replay) */
+    PGW_BATCH_BALANCE = 2   /* per-job round quotas by modelled cost (else 1 each) */
+};
+
+/* nstreams = 0 picks 8; flags: pgw_batch_flags. */
+int pgw_batch_create(pgw_ctx* ctx, uint32_t nstreams, uint32_t flags, pgw_batch** out);
 /* Adds a job: h_v (N x b) is copied, seq[0] = V^T V is computed at once.
  * An operator may carry at most one unfinished job.  *job receives its id. */
 int pgw_batch_add(pgw_batch* b, pgw_op* op, const uint32_t* h_v, uint64_t max_len,
                   uint64_t* job);
-/* Advances every unfinished job by min(max_rounds (0 = 1), rounds the
- * closest-to-finished job still needs) rounds; *active (optional)
- * receives the number of jobs still unfinished.  Returns when the rounds are done. */
-int pgw_batch_step(pgw_batch* b, uint32_t max_rounds, uint64_t* active);
+/* Runs up to max_replays (0 = 1) replays, fewer where a job would pass its
+ * end; *active (optional) receives the number of unfinished jobs.  Returns
+ * when the replays are done. */
+int pgw_batch_step(pgw_batch* b, uint32_t max_replays, uint64_t* active);
 int pgw_batch_poll(pgw_batch* b, uint64_t job, int* done);
 /* Copies a finished job's L x b(b+1)/2 sequence and final N x b block to
  * the host (same layout as pgw_sequence_run), frees its device state and
@@ -237,21 +247,20 @@ int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_cu
 /* Drops a job (finished or not) and frees its device state. */
 int pgw_batch_cancel(pgw_batch* b, uint64_t job);
 /* Number of graph captures so far (instrumentation). */
-int pgw_batch_graph_builds(const pgw_batch* b, uint64_t* builds);
+int pgw_batch_graph_builds(pgw_batch* b, uint64_t* builds);
 int pgw_batch_destroy(pgw_batch* b);
 
 /* ------------------------------------------------------------------------
- * Serving: a worker thread that admits submitted sector jobs by their
- * device-memory footprint and runs them as one continuously refilled batch.
+ * Serving: sector jobs admitted by their device-memory footprint and run
+ * as one continuously refilled, cost-balanced batch.
  *
  * Replaces the reference's resource-pool scheduling of sector processes
  * (scripts/reproduce.py:78-111, 246-279) for sector jobs.  Admission is
- * FIFO: the head of the queue is admitted when its estimated footprint fits
- * in `hbm_budget` bytes next to the running jobs and fewer than
- * `max_active` jobs run; a job whose footprint exceeds the whole budget
- * fails at once.  Between batch steps of `rounds_per_step` rounds finished
- * jobs leave and queued jobs join.  Results of a job do not depend on what
- * else runs with it.
+ * FIFO: the head of the queue is admitted when its footprint fits in
+ * `hbm_budget` bytes next to the running jobs and fewer than `max_active`
+ * jobs run; a job whose footprint exceeds the whole budget fails at once.
+ * Between batch steps finished jobs leave and admitted jobs join.  Results
+ * of a job do not depend on what else runs with it.
  * ---------------------------------------------------------------------- */
 typedef struct pgw_server pgw_server;
 
@@ -275,11 +284,14 @@ int pgw_server_submit(pgw_server* s, const pgw_operator_desc* desc, const uint32
  * 0 queued, 1 running, 2 done, 3 failed. */
 int pgw_server_wait(pgw_server* s, uint64_t ticket, double timeout_s, int* state);
 /* Copies a finished job's sequence (L x b(b+1)/2) and final block (N x b)
- * (either pointer may be null) and forgets the ticket.  times3 (optional)
- * receives submit, admit and finish times in seconds since the server
- * started.  A failed job returns PGW_ERR_INVALID with its error message. */
+ * (either pointer may be null) and forgets the ticket.  times (optional, 5
+ * entries) receives, in seconds since the server started: submitted,
+ * admitted, in the batch (operator built, V uploaded), out of the batch
+ * (last round done), finished (results in host memory).  A failed job
+ * returns PGW_ERR_INVALID with its error message. */
 int pgw_server_result(pgw_server* s, uint64_t ticket, uint32_t* h_seq, uint32_t* h_curv,
-                      uint64_t* len, double* times3);
+                      uint64_t* len, double* times);
+/* completed counts finished and failed jobs. */
 int pgw_server_stats(pgw_server* s, uint64_t* completed, uint64_t* peak_bytes,
                      uint64_t* peak_active, uint64_t* queued);
 /* Finishes every queued and running job, then stops the worker. */

@@ -132,7 +132,7 @@ def lib():
         "pgw_bench_apply": (I, [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u32)]),
         "pgw_operator_autotune": (I, [vp, u32, vp, vp]),
-        "pgw_batch_create": (I, [vp, u32, I, ctypes.POINTER(vp)]),
+        "pgw_batch_create": (I, [vp, u32, u32, ctypes.POINTER(vp)]),
         "pgw_batch_add": (I, [vp, vp, vp, u64, ctypes.POINTER(u64)]),
         "pgw_batch_step": (I, [vp, u32, ctypes.POINTER(u64)]),
         "pgw_batch_poll": (I, [vp, u64, ctypes.POINTER(I)]),
@@ -423,10 +423,14 @@ class Operator:
 class Batch:
     """Many sector sequence loops advanced together (pgw_batch_*)."""
 
-    def __init__(self, ctx: Context, nstreams: int = 8, use_graph: bool = True):
+    GRAPH, BALANCE = 1, 2
+
+    def __init__(self, ctx: Context, nstreams: int = 8, use_graph: bool = True,
+                 balance: bool = False):
         self.ctx = ctx
         h = ctypes.c_void_p()
-        _check(lib().pgw_batch_create(ctx.h, nstreams, 1 if use_graph else 0, ctypes.byref(h)))
+        flags = (self.GRAPH if use_graph else 0) | (self.BALANCE if balance else 0)
+        _check(lib().pgw_batch_create(ctx.h, nstreams, flags, ctypes.byref(h)))
         self.h = h
         self.jobs = {}
 
@@ -437,9 +441,10 @@ class Batch:
         self.jobs[job.value] = (op, max_len)
         return job.value
 
-    def step(self, max_rounds: int = 1) -> int:
+    def step(self, max_replays: int = 1) -> int:
+        """Up to max_replays replays; returns the number of unfinished jobs."""
         active = ctypes.c_uint64()
-        _check(lib().pgw_batch_step(self.h, max_rounds, ctypes.byref(active)))
+        _check(lib().pgw_batch_step(self.h, max_replays, ctypes.byref(active)))
         return active.value
 
     def done(self, job: int) -> bool:
@@ -459,9 +464,9 @@ class Batch:
         assert n.value == L
         return seq.reshape(L, ut), curv
 
-    def run(self, max_rounds: int = 0):
+    def run(self, max_replays: int = 0):
         """Step until every job is done; returns {job: (seq, curv)}."""
-        while self.step(max_rounds or (1 << 30)):
+        while self.step(max_replays or (1 << 30)):
             pass
         return {j: self.result(j) for j in list(self.jobs)}
 
@@ -495,10 +500,10 @@ class Server:
     QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
     def __init__(self, ctx: Context, hbm_budget: int, max_active: int = 16,
-                 rounds_per_step: int = 8):
+                 replays_per_step: int = 1):
         self.ctx = ctx
         h = ctypes.c_void_p()
-        _check(lib().pgw_server_create(ctx.h, hbm_budget, max_active, rounds_per_step,
+        _check(lib().pgw_server_create(ctx.h, hbm_budget, max_active, replays_per_step,
                                        ctypes.byref(h)))
         self.h = h
         self.meta = {}
@@ -522,13 +527,14 @@ class Server:
         return st.value
 
     def result(self, ticket: int):
-        """(seq, curv, (t_submit, t_admit, t_finish)) of a finished job."""
+        """(seq, curv, times) of a finished job; times = seconds since the server
+        started at: submitted, admitted, in the batch, out of the batch, finished."""
         b, cols, max_len = self.meta.pop(ticket)
         ut = b * (b + 1) // 2
         L = sequence_length(max_len)
         seq = np.empty(L * ut, np.uint32)
         curv = np.empty(cols * b, np.uint32)
-        times = np.zeros(3, np.float64)
+        times = np.zeros(5, np.float64)
         n = ctypes.c_uint64()
         _check(lib().pgw_server_result(self.h, ticket, _ptr(seq), _ptr(curv), ctypes.byref(n),
                                        _ptr(times)))

@@ -4,61 +4,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
-#include "pgw_kernels.h"
+#include "pgw_op.h"
 
 using namespace pgw;
 
 namespace {
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        return PGW_OK;
-    } catch (const Error& e) {
-        set_last_error(e.what());
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        set_last_error("host allocation failed");
-        return PGW_ERR_MEMORY;
-    } catch (const std::exception& e) {
-        set_last_error(e.what());
-        return PGW_ERR_INVALID;
-    }
-}
-
-struct DeviceArena {
-    std::vector<void*> ptrs;
-    uint64_t bytes = 0;
-    template <typename T>
-    T* alloc(uint64_t count) {
-        if (count == 0) count = 1;
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, count * sizeof(T));
-        if (e != hipSuccess)
-            throw Error(PGW_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
-                                            " bytes failed: " + hipGetErrorString(e));
-        ptrs.push_back(p);
-        bytes += count * sizeof(T);
-        return (T*)p;
-    }
-    template <typename T>
-    T* upload(const T* host, uint64_t count) {
-        T* d = alloc<T>(count);
-        if (count) PGW_HIP(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
-        return d;
-    }
-    template <typename T>
-    T* upload(const std::vector<T>& v) { return upload(v.data(), (uint64_t)v.size()); }
-    void release() {
-        for (void* p : ptrs) (void)hipFree(p);
-        ptrs.clear();
-        bytes = 0;
-    }
-};
 
 PhiDevice upload_phi(DeviceArena& A, const PhiHost& h) {
     PhiDevice d;
@@ -93,51 +47,6 @@ PhiDevice upload_phi(DeviceArena& A, const PhiHost& h) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-
-struct pgw_op {
-    pgw_ctx* ctx = nullptr;
-    uint32_t kind = 0, p = 0, b = 0, method = PGW_METHOD_PLAIN;
-    ModP m{};
-    uint64_t rows = 0, cols = 0, mid = 0;
-    DeviceArena arena;
-    uint32_t* d_dr = nullptr;
-    uint32_t* d_dc = nullptr;
-    uint32_t* d_negdc = nullptr;
-    pgw_rowmix_pair* d_pairs = nullptr;
-    uint32_t rounds = 0;
-    uint64_t pair_count = 0;
-    int64_t unpaired = -1;
-    uint32_t* d_scaled = nullptr;  // cols x b
-    uint32_t* d_mid = nullptr;     // mid x b
-    uint32_t* d_target = nullptr;  // rows x b
-    uint64_t* d_gram = nullptr;
-    // synthetic
-    uint32_t n = 0, k = 0, M = 0, Mp = 0;
-    uint64_t u_src = 0, u_tgt = 0;
-    uint32_t* d_tab[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
-    uint32_t k4[4] = {0, 0, 0, 0};
-    MfmaTable mt[4];
-    int mfma_cfg[4] = {1, 1, 1, 1};  // tile configuration per sweep (pgw_operator_autotune)
-    uint32_t nl = 0;
-    int8_t* d_lp_src = nullptr;
-    int8_t* d_lp_mid = nullptr;
-    // cyclic eliminated
-    PhiDevice dm, d2;
-    uint32_t* d_zoff = nullptr;
-    uint32_t* d_zrank = nullptr;
-    uint16_t* d_zkappa = nullptr;
-    uint32_t* d_R = nullptr;
-    uint32_t* d_Bk = nullptr;
-    uint32_t n_v = 0, kdim = 0;
-    uint64_t x_cols = 0, z_count = 0;
-    uint32_t* d_t = nullptr;
-    uint32_t* d_y = nullptr;
-    uint32_t* d_h = nullptr;
-    uint32_t* d_tadj = nullptr;
-    // timing of the sweep kernels (pgw_bench_apply)
-    bool profile = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> sweep_events;
-};
 
 namespace {
 
@@ -242,12 +151,17 @@ void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
     }
 }
 
-void apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st) {
+
+}  // namespace
+
+void pgw::apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st) {
     forward_into_target(op, in, st);
     launch_target_precond(op->d_target, op->d_pairs, op->pair_count, op->rounds, op->d_dr,
                           op->unpaired, op->rows, op->b, op->m, st);
     backward_from_target(op, out, st);
 }
+
+namespace {
 
 hipStream_t pick_stream(pgw_op* op, void* stream) {
     return stream ? (hipStream_t)stream : op->ctx->stream;
@@ -338,8 +252,9 @@ int pgw_ctx_device_info(pgw_ctx* ctx, char* name, size_t name_len, uint64_t* tot
         hipDeviceProp_t prop;
         PGW_HIP(hipGetDeviceProperties(&prop, ctx->device));
         if (name && name_len) {
-            std::strncpy(name, prop.name, name_len - 1);
-            name[name_len - 1] = 0;
+            // the marketing name can be empty where the driver's id table is missing
+            std::snprintf(name, name_len, "%s%s%s, %d CUs", prop.name, prop.name[0] ? " " : "",
+                          prop.gcnArchName, prop.multiProcessorCount);
         }
         if (total_bytes) *total_bytes = prop.totalGlobalMem;
     });
@@ -374,6 +289,7 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
         require(h_row_precond && h_col_precond, "preconditioners are required");
         PGW_HIP(hipSetDevice(ctx->device));
         op = new pgw_op;
+        op->arena.lock = &ctx->capture_mtx;
         op->ctx = ctx;
         op->kind = desc->kind;
         op->p = desc->prime;
@@ -413,7 +329,10 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
                 for (int w = 0; w < 4; ++w) {
                     DenseSweep s = sweep_desc(op, w);
                     check_mfma_bound(s, op->nl);
-                    build_mfma_table(op->mt[w], op->n, s.r_out, s.k_in, op->p, src[w], !s.fwd);
+                    {
+                        std::lock_guard<std::mutex> cl(ctx->capture_mtx);
+                        build_mfma_table(op->mt[w], op->n, s.r_out, s.k_in, op->p, src[w], !s.fwd);
+                    }
                     A.ptrs.push_back(op->mt[w].d);
                     A.bytes += (uint64_t)op->n * op->nl * op->mt[w].rows_pad * op->mt[w].k_pad;
                 }
@@ -472,10 +391,11 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             throw Error(PGW_ERR_INVALID, "unknown operator kind");
         }
         // preconditioners and the row-mixing plan
-        for (uint64_t q = 0; q < op->rows; ++q)
-            require(h_row_precond[q] >= 1 && h_row_precond[q] < op->p, "row preconditioner entries must be nonzero residues");
-        for (uint64_t q = 0; q < op->cols; ++q)
-            require(h_col_precond[q] >= 1 && h_col_precond[q] < op->p, "column preconditioner entries must be nonzero residues");
+        uint32_t lo, hi;
+        min_max_u32(h_row_precond, op->rows, lo, hi);
+        require(op->rows == 0 || (lo >= 1 && hi < op->p), "row preconditioner entries must be nonzero residues");
+        min_max_u32(h_col_precond, op->cols, lo, hi);
+        require(op->cols == 0 || (lo >= 1 && hi < op->p), "column preconditioner entries must be nonzero residues");
         op->d_dr = A.upload(h_row_precond, op->rows);
         op->d_dc = A.upload(h_col_precond, op->cols);
         std::vector<uint32_t> negdc(op->cols);
@@ -522,12 +442,15 @@ int pgw_operator_destroy(pgw_op* op) {
     return guarded([&] {
         if (!op) return;
         PGW_HIP(hipSetDevice(op->ctx->device));
-        PGW_HIP(hipStreamSynchronize(op->ctx->stream));
-        op->arena.release();
-        for (auto& e : op->sweep_events) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
+        {
+            std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
+            PGW_HIP(hipStreamSynchronize(op->ctx->stream));
+            for (auto& e : op->sweep_events) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
         }
+        op->arena.release();
         delete op;
     });
 }
@@ -587,7 +510,11 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
         const uint64_t N = op->cols, b = op->b, ut = b * (b + 1) / 2;
         const uint64_t rounds = (max_len - 1 + 3) / 4;
         const uint64_t L = 1 + 4 * rounds;
-        for (uint64_t q = 0; q < N * b; ++q) require(h_v[q] < op->p, "V entries must be reduced modulo p");
+        {
+            uint32_t lo, hi;
+            min_max_u32(h_v, N * b, lo, hi);
+            require(N * b == 0 || hi < op->p, "V entries must be reduced modulo p");
+        }
         PGW_HIP(hipMalloc(&d_B, N * b * sizeof(uint32_t)));
         PGW_HIP(hipMalloc(&d_D, N * b * sizeof(uint32_t)));
         PGW_HIP(hipMalloc(&d_seq, L * ut * sizeof(uint32_t)));
@@ -747,606 +674,6 @@ int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, do
         if (sweeps_per_apply) *sweeps_per_apply = (uint32_t)count;
         (void)hipEventDestroy(t0);
         (void)hipEventDestroy(t1);
-    });
-}
-
-}  // extern "C"
-
-// ===========================================================================
-// batches: many sector jobs advanced together, one captured graph per round
-// ===========================================================================
-
-#include <chrono>
-#include <condition_variable>
-#include <deque>
-#include <map>
-#include <mutex>
-#include <thread>
-
-namespace {
-
-struct BatchJob {
-    pgw_op* op = nullptr;
-    uint32_t* d_B = nullptr;
-    uint32_t* d_D = nullptr;
-    uint32_t* d_seq = nullptr;
-    uint32_t* d_pos = nullptr;
-    uint64_t L = 0, rounds_total = 0, rounds_done = 0;
-    bool done = false;
-    uint64_t bytes = 0;
-
-    void release() {
-        for (uint32_t* p : {d_B, d_D, d_seq, d_pos})
-            if (p) (void)hipFree(p);
-        d_B = d_D = d_seq = d_pos = nullptr;
-    }
-};
-
-// One round of the sequence loop for a job (cuprym_cyclic.cu:2439-2449):
-// D = S B; seq[pos] = B^T D, seq[pos+1] = D^T D; B = S D; seq[pos+2] = B^T D,
-// seq[pos+3] = B^T B; pos += 4.  Positions are read on the device.
-void enqueue_round(BatchJob& j, hipStream_t st) {
-    pgw_op* op = j.op;
-    const uint64_t ut = (uint64_t)op->b * (op->b + 1) / 2;
-    apply_square_impl(op, j.d_B, j.d_D, st);
-    launch_gram(j.d_B, j.d_D, op->cols, op->b, 1, j.d_seq, j.d_seq + ut, op->d_gram, op->m, st,
-                j.d_pos, 0);
-    apply_square_impl(op, j.d_D, j.d_B, st);
-    launch_gram(j.d_B, j.d_D, op->cols, op->b, 2, j.d_seq, j.d_seq + ut, op->d_gram, op->m, st,
-                j.d_pos, 2);
-    launch_advance_pos(j.d_pos, 4, st);
-}
-
-}  // namespace
-
-struct pgw_batch {
-    pgw_ctx* ctx = nullptr;
-    std::vector<hipStream_t> streams;
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> joins;
-    std::map<uint64_t, BatchJob> jobs;
-    uint64_t next_id = 1;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    std::vector<uint64_t> graph_jobs;  // active set the graph was captured for
-    bool use_graph = true;
-    uint64_t graph_builds = 0;
-
-    void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-        graph_jobs.clear();
-    }
-
-    std::vector<uint64_t> active() const {
-        std::vector<uint64_t> a;
-        for (const auto& kv : jobs)
-            if (!kv.second.done) a.push_back(kv.first);
-        return a;
-    }
-
-    // enqueue one round of every active job, spread over the streams, joined on streams[0]
-    void enqueue_all(const std::vector<uint64_t>& act) {
-        hipStream_t s0 = streams[0];
-        PGW_HIP(hipEventRecord(fork, s0));
-        const size_t S = streams.size();
-        for (size_t q = 1; q < S && q < act.size(); ++q) PGW_HIP(hipStreamWaitEvent(streams[q], fork, 0));
-        for (size_t q = 0; q < act.size(); ++q) enqueue_round(jobs.at(act[q]), streams[q % S]);
-        for (size_t q = 1; q < S && q < act.size(); ++q) {
-            PGW_HIP(hipEventRecord(joins[q], streams[q]));
-            PGW_HIP(hipStreamWaitEvent(s0, joins[q], 0));
-        }
-    }
-
-    void step(uint32_t max_rounds) {
-        std::vector<uint64_t> act = active();
-        if (act.empty()) return;
-        uint64_t r = max_rounds ? max_rounds : 1;
-        for (uint64_t id : act) r = std::min(r, jobs.at(id).rounds_total - jobs.at(id).rounds_done);
-        hipStream_t s0 = streams[0];
-        if (use_graph) {
-            if (act != graph_jobs) {
-                drop_graph();
-                PGW_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-                try {
-                    enqueue_all(act);
-                } catch (...) {
-                    hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(s0, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                PGW_HIP(hipStreamEndCapture(s0, &graph));
-                PGW_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                graph_jobs = act;
-                ++graph_builds;
-            }
-            for (uint64_t q = 0; q < r; ++q) PGW_HIP(hipGraphLaunch(exec, s0));
-        } else {
-            for (uint64_t q = 0; q < r; ++q) enqueue_all(act);
-        }
-        PGW_HIP(hipStreamSynchronize(s0));
-        for (uint64_t id : act) {
-            BatchJob& j = jobs.at(id);
-            j.rounds_done += r;
-            if (j.rounds_done >= j.rounds_total) j.done = true;
-        }
-    }
-};
-
-extern "C" {
-
-int pgw_batch_create(pgw_ctx* ctx, uint32_t nstreams, int use_graph, pgw_batch** out) {
-    pgw_batch* b = nullptr;
-    int rc = guarded([&] {
-        require(ctx && out, "null context or output pointer");
-        PGW_HIP(hipSetDevice(ctx->device));
-        b = new pgw_batch;
-        b->ctx = ctx;
-        b->use_graph = use_graph != 0;
-        if (nstreams == 0) nstreams = 8;
-        b->streams.resize(nstreams, nullptr);
-        b->joins.resize(nstreams, nullptr);
-        for (uint32_t q = 0; q < nstreams; ++q) {
-            PGW_HIP(hipStreamCreateWithFlags(&b->streams[q], hipStreamNonBlocking));
-            PGW_HIP(hipEventCreateWithFlags(&b->joins[q], hipEventDisableTiming));
-        }
-        PGW_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-        *out = b;
-    });
-    if (rc != PGW_OK && b) {
-        for (auto s : b->streams) if (s) (void)hipStreamDestroy(s);
-        for (auto e : b->joins) if (e) (void)hipEventDestroy(e);
-        if (b->fork) (void)hipEventDestroy(b->fork);
-        delete b;
-    }
-    return rc;
-}
-
-int pgw_batch_add(pgw_batch* b, pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint64_t* job) {
-    BatchJob j;
-    int rc = guarded([&] {
-        require(b && op && h_v && job, "null batch, operator or buffer");
-        require(op->ctx == b->ctx, "operator belongs to another context");
-        require(max_len >= 1, "sequence length must be positive");
-        for (const auto& kv : b->jobs)
-            require(kv.second.done || kv.second.op != op, "an operator can carry one active job at a time");
-        PGW_HIP(hipSetDevice(b->ctx->device));
-        const uint64_t N = op->cols, bb = op->b, ut = bb * (bb + 1) / 2;
-        for (uint64_t q = 0; q < N * bb; ++q) require(h_v[q] < op->p, "V entries must be reduced modulo p");
-        j.op = op;
-        j.rounds_total = (max_len - 1 + 3) / 4;
-        j.L = 1 + 4 * j.rounds_total;
-        PGW_HIP(hipMalloc(&j.d_B, N * bb * sizeof(uint32_t)));
-        PGW_HIP(hipMalloc(&j.d_D, N * bb * sizeof(uint32_t)));
-        PGW_HIP(hipMalloc(&j.d_seq, j.L * ut * sizeof(uint32_t)));
-        PGW_HIP(hipMalloc(&j.d_pos, sizeof(uint32_t)));
-        j.bytes = 2 * N * bb * sizeof(uint32_t) + j.L * ut * sizeof(uint32_t);
-        hipStream_t s0 = b->streams[0];
-        const uint32_t one = 1;
-        PGW_HIP(hipMemcpyAsync(j.d_B, h_v, N * bb * sizeof(uint32_t), hipMemcpyHostToDevice, s0));
-        PGW_HIP(hipMemcpyAsync(j.d_pos, &one, sizeof(uint32_t), hipMemcpyHostToDevice, s0));
-        launch_gram(j.d_B, j.d_B, N, op->b, 0, j.d_seq, nullptr, op->d_gram, op->m, s0);  // seq[0] = V^T V
-        PGW_HIP(hipStreamSynchronize(s0));
-        if (j.rounds_total == 0) j.done = true;
-        const uint64_t id = b->next_id++;
-        b->jobs.emplace(id, j);
-        *job = id;
-    });
-    if (rc != PGW_OK) j.release();
-    return rc;
-}
-
-int pgw_batch_step(pgw_batch* b, uint32_t max_rounds, uint64_t* active) {
-    return guarded([&] {
-        require(b != nullptr, "null batch");
-        PGW_HIP(hipSetDevice(b->ctx->device));
-        b->step(max_rounds);
-        if (active) *active = b->active().size();
-    });
-}
-
-int pgw_batch_poll(pgw_batch* b, uint64_t job, int* done) {
-    return guarded([&] {
-        require(b && done, "null batch or output");
-        auto it = b->jobs.find(job);
-        require(it != b->jobs.end(), "unknown batch job");
-        *done = it->second.done ? 1 : 0;
-    });
-}
-
-int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_curv, uint64_t* len) {
-    return guarded([&] {
-        require(b && h_seq && h_curv, "null batch or buffer");
-        auto it = b->jobs.find(job);
-        require(it != b->jobs.end(), "unknown batch job");
-        BatchJob& j = it->second;
-        require(j.done, "batch job has not finished");
-        PGW_HIP(hipSetDevice(b->ctx->device));
-        const uint64_t N = j.op->cols, bb = j.op->b, ut = bb * (bb + 1) / 2;
-        PGW_HIP(hipMemcpy(h_seq, j.d_seq, j.L * ut * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        PGW_HIP(hipMemcpy(h_curv, j.d_B, N * bb * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        if (len) *len = j.L;
-        j.release();
-        b->jobs.erase(it);
-    });
-}
-
-int pgw_batch_cancel(pgw_batch* b, uint64_t job) {
-    return guarded([&] {
-        require(b != nullptr, "null batch");
-        auto it = b->jobs.find(job);
-        require(it != b->jobs.end(), "unknown batch job");
-        PGW_HIP(hipSetDevice(b->ctx->device));
-        for (auto s : b->streams) PGW_HIP(hipStreamSynchronize(s));
-        b->drop_graph();
-        it->second.release();
-        b->jobs.erase(it);
-    });
-}
-
-int pgw_batch_destroy(pgw_batch* b) {
-    return guarded([&] {
-        if (!b) return;
-        PGW_HIP(hipSetDevice(b->ctx->device));
-        for (auto s : b->streams) (void)hipStreamSynchronize(s);
-        b->drop_graph();
-        for (auto& kv : b->jobs) kv.second.release();
-        for (auto s : b->streams) (void)hipStreamDestroy(s);
-        for (auto e : b->joins) (void)hipEventDestroy(e);
-        (void)hipEventDestroy(b->fork);
-        delete b;
-    });
-}
-
-int pgw_batch_graph_builds(const pgw_batch* b, uint64_t* builds) {
-    return guarded([&] {
-        require(b && builds, "null batch or output");
-        *builds = b->graph_builds;
-    });
-}
-
-}  // extern "C"
-
-// ===========================================================================
-// serving: a device-resident loop that admits sector jobs by HBM footprint
-// ===========================================================================
-
-namespace {
-
-using Clock = std::chrono::steady_clock;
-
-struct OwnedDesc {
-    pgw_operator_desc d{};
-    std::vector<uint32_t> table_d, table_e, mu, right_inverse, kernel_basis;
-    std::vector<uint16_t> wv, wa0, wa1, kw;
-
-    void copy_from(const pgw_operator_desc& s) {
-        d = s;
-        if (s.kind == PGW_KIND_SYNTHETIC) {
-            const uint64_t cnt = (uint64_t)s.n * s.m_tgt * s.m_src;
-            table_d.assign(s.table_d, s.table_d + cnt);
-            table_e.assign(s.table_e, s.table_e + cnt);
-            d.table_d = table_d.data();
-            d.table_e = table_e.data();
-        } else {
-            const uint64_t n = s.genus - 3;
-            auto grab = [](auto& dst, const auto* src, uint64_t cnt) {
-                if (src && cnt) dst.assign(src, src + cnt);
-            };
-            grab(mu, s.mu, n * n * s.genus);
-            grab(wv, s.weight_v, n);
-            grab(wa0, s.weight_a0, s.genus);
-            grab(wa1, s.weight_a1, n);
-            grab(kw, s.kernel_weights, s.kernel_dim);
-            grab(right_inverse, s.right_inverse, (uint64_t)s.genus * n);
-            grab(kernel_basis, s.kernel_basis, (uint64_t)s.genus * s.kernel_dim);
-            d.mu = mu.data();
-            d.weight_v = wv.data();
-            d.weight_a0 = wa0.data();
-            d.weight_a1 = wa1.data();
-            d.kernel_weights = kw.data();
-            d.right_inverse = right_inverse.data();
-            d.kernel_basis = kernel_basis.data();
-        }
-    }
-};
-
-struct ServerJob {
-    uint64_t ticket = 0;
-    OwnedDesc desc;
-    std::vector<uint32_t> dr, dc, v;
-    std::vector<pgw_rowmix_pair> pairs;
-    uint32_t rounds = 0;
-    uint64_t max_len = 0, rows = 0, cols = 0, estimate = 0;
-    pgw_op* op = nullptr;
-    uint64_t batch_job = 0;
-    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
-    std::string error;
-    std::vector<uint32_t> seq, curv;
-    uint64_t L = 0;
-    Clock::time_point t_submit, t_admit, t_finish;
-};
-
-// device bytes a job needs: operator (tables, digit planes, scratch blocks) plus
-// its sequence state (two N x b blocks and L x b(b+1)/2 terms)
-uint64_t estimate_job_bytes(const pgw_operator_desc& d, uint64_t rows, uint64_t cols,
-                            uint64_t max_len, uint32_t rowmix_rounds) {
-    const uint64_t b = d.block, ut = b * (b + 1) / 2;
-    const uint64_t L = 1 + 4 * ((max_len - 1 + 3) / 4);
-    uint64_t bytes = 2 * cols * b * 4 + L * ut * 4;            // job state
-    bytes += (rows + 2 * cols) * 4 + rows * b * 4;             // Dr, Dc, -Dc, target block
-    bytes += gram_scratch_bytes((uint32_t)b) + rows / 2 * 16 * rowmix_rounds;  // Gram scratch, pairs
-    if (d.kind == PGW_KIND_SYNTHETIC) {
-        const uint64_t mid = host_binom(d.n, d.k - 1) * d.m_tgt;
-        const uint64_t us = host_binom(d.n, d.k), ut2 = host_binom(d.n, d.k - 1);
-        bytes += mid * b * 4 + cols * b * 4;                   // mid block, scaled block
-        const uint64_t rp = (std::max(d.m_src, d.m_tgt) + 15) / 16 * 16 + 16;
-        const uint64_t kp = (std::max(d.m_src, d.m_tgt) + 31) / 32 * 32 + 32;
-        bytes += 4 * (uint64_t)d.n * 4 * rp * kp * 2;          // tables (either path, upper bound)
-        bytes += 2 * us * 4 * b * (((uint64_t)d.m_src + 15) / 16 * 16);   // source digit planes
-        bytes += 2 * ut2 * 4 * b * (((uint64_t)d.m_tgt + 15) / 16 * 16);  // mid digit planes
-    } else {
-        bytes += 6 * (rows + cols) * b * 4 + 64ull * (rows + cols);
-    }
-    return bytes + (16ull << 20);
-}
-
-}  // namespace
-
-struct pgw_server {
-    pgw_ctx* ctx = nullptr;
-    uint64_t budget = 0;
-    uint32_t max_active = 0, rounds_per_step = 1;
-    pgw_batch* batch = nullptr;
-    std::mutex mtx;
-    std::condition_variable cv_work, cv_done;
-    std::deque<uint64_t> queue;
-    std::map<uint64_t, std::unique_ptr<ServerJob>> jobs;
-    std::vector<uint64_t> running;
-    uint64_t next_ticket = 1, in_use = 0, peak_bytes = 0, peak_active = 0, completed = 0;
-    bool stop = false;
-    std::string fatal;
-    std::thread worker;
-    Clock::time_point t0 = Clock::now();
-
-    void finish_job(ServerJob& j, int state, const std::string& err) {
-        j.state = state;
-        j.error = err;
-        j.t_finish = Clock::now();
-        if (j.op) {
-            pgw_operator_destroy(j.op);
-            j.op = nullptr;
-        }
-    }
-
-    void run() {
-        (void)hipSetDevice(ctx->device);
-        for (;;) {
-            // admission: FIFO while the HBM budget and the slot count allow
-            std::vector<ServerJob*> admit;
-            {
-                std::unique_lock<std::mutex> lk(mtx);
-                if (running.empty())
-                    cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-                if (stop && queue.empty() && running.empty()) return;
-                while (!queue.empty() && running.size() + admit.size() < max_active) {
-                    ServerJob& j = *jobs.at(queue.front());
-                    if (j.estimate > budget) {
-                        queue.pop_front();
-                        finish_job(j, 3, "job needs more device memory than the server budget");
-                        cv_done.notify_all();
-                        continue;
-                    }
-                    if (in_use + j.estimate > budget) break;
-                    in_use += j.estimate;
-                    peak_bytes = std::max(peak_bytes, in_use);
-                    queue.pop_front();
-                    admit.push_back(&j);
-                }
-            }
-            for (ServerJob* j : admit) {
-                j->t_admit = Clock::now();
-                int rc = pgw_operator_create(ctx, &j->desc.d, j->dr.data(), j->dc.data(), j->rounds,
-                                             j->pairs.empty() ? nullptr : j->pairs.data(), &j->op);
-                if (rc == PGW_OK)
-                    rc = pgw_batch_add(batch, j->op, j->v.data(), j->max_len, &j->batch_job);
-                std::lock_guard<std::mutex> lk(mtx);
-                if (rc != PGW_OK) {
-                    in_use -= j->estimate;
-                    finish_job(*j, 3, pgw_last_error());
-                    cv_done.notify_all();
-                } else {
-                    // replace the estimate by what the operator and the job really hold
-                    pgw_op_info info{};
-                    pgw_operator_info(j->op, &info);
-                    const uint64_t b = j->desc.d.block, ut = b * (b + 1) / 2;
-                    const uint64_t L = 1 + 4 * ((j->max_len - 1 + 3) / 4);
-                    const uint64_t actual = info.device_bytes + 2 * j->cols * b * 4 + L * ut * 4;
-                    in_use = in_use - j->estimate + actual;
-                    j->estimate = actual;
-                    peak_bytes = std::max(peak_bytes, in_use);
-                    running.push_back(j->ticket);
-                    peak_active = std::max<uint64_t>(peak_active, running.size());
-                }
-                j->dr = std::vector<uint32_t>();
-                j->dc = std::vector<uint32_t>();
-                j->v = std::vector<uint32_t>();
-                j->pairs = std::vector<pgw_rowmix_pair>();
-            }
-            if (running.empty()) continue;
-            uint64_t active = 0;
-            if (pgw_batch_step(batch, rounds_per_step, &active) != PGW_OK) {
-                std::lock_guard<std::mutex> lk(mtx);
-                fatal = pgw_last_error();
-                for (uint64_t t : running) {
-                    pgw_batch_cancel(batch, jobs.at(t)->batch_job);
-                    finish_job(*jobs.at(t), 3, fatal);
-                    in_use -= jobs.at(t)->estimate;
-                }
-                running.clear();
-                cv_done.notify_all();
-                continue;
-            }
-            // completions: stream results to host memory, free the device state
-            std::vector<uint64_t> still;
-            for (uint64_t t : running) {
-                ServerJob& j = *jobs.at(t);
-                int done = 0;
-                pgw_batch_poll(batch, j.batch_job, &done);
-                if (!done) {
-                    still.push_back(t);
-                    continue;
-                }
-                const uint64_t b = j.desc.d.block, ut = b * (b + 1) / 2;
-                const uint64_t L = 1 + 4 * ((j.max_len - 1 + 3) / 4);
-                j.seq.resize(L * ut);
-                j.curv.resize(j.cols * b);
-                int rc = pgw_batch_result(batch, j.batch_job, j.seq.data(), j.curv.data(), &j.L);
-                std::lock_guard<std::mutex> lk(mtx);
-                in_use -= j.estimate;
-                finish_job(j, rc == PGW_OK ? 2 : 3, rc == PGW_OK ? "" : pgw_last_error());
-                ++completed;
-                cv_done.notify_all();
-            }
-            std::lock_guard<std::mutex> lk(mtx);
-            running.swap(still);
-        }
-    }
-};
-
-extern "C" {
-
-int pgw_job_footprint(const pgw_operator_desc* desc, uint64_t max_len, uint32_t rowmix_rounds,
-                      uint64_t* bytes) {
-    return guarded([&] {
-        require(desc && bytes, "null description or output");
-        require(max_len >= 1, "sequence length must be positive");
-        uint64_t rows = 0, cols = 0;
-        if (pgw_operator_dims(desc, &rows, &cols) != PGW_OK) throw Error(PGW_ERR_INVALID, pgw_last_error());
-        *bytes = estimate_job_bytes(*desc, rows, cols, max_len, rowmix_rounds);
-    });
-}
-
-int pgw_server_create(pgw_ctx* ctx, uint64_t hbm_budget, uint32_t max_active,
-                      uint32_t rounds_per_step, pgw_server** out) {
-    pgw_server* s = nullptr;
-    int rc = guarded([&] {
-        require(ctx && out, "null context or output pointer");
-        require(hbm_budget > 0 && max_active > 0, "budget and slot count must be positive");
-        s = new pgw_server;
-        s->ctx = ctx;
-        s->budget = hbm_budget;
-        s->max_active = max_active;
-        s->rounds_per_step = rounds_per_step ? rounds_per_step : 1;
-        int brc = pgw_batch_create(ctx, std::min<uint32_t>(max_active, 8), 1, &s->batch);
-        if (brc != PGW_OK) throw Error(brc, pgw_last_error());
-        s->worker = std::thread([s] { s->run(); });
-        *out = s;
-    });
-    if (rc != PGW_OK && s) {
-        if (s->batch) pgw_batch_destroy(s->batch);
-        delete s;
-    }
-    return rc;
-}
-
-int pgw_server_submit(pgw_server* s, const pgw_operator_desc* desc, const uint32_t* h_dr,
-                      const uint32_t* h_dc, uint32_t rowmix_rounds, const pgw_rowmix_pair* h_pairs,
-                      const uint32_t* h_v, uint64_t max_len, uint64_t* ticket) {
-    return guarded([&] {
-        require(s && desc && h_dr && h_dc && h_v && ticket, "null server, description or buffer");
-        require(max_len >= 1, "sequence length must be positive");
-        uint64_t rows = 0, cols = 0;
-        if (pgw_operator_dims(desc, &rows, &cols) != PGW_OK) throw Error(PGW_ERR_INVALID, pgw_last_error());
-        auto j = std::make_unique<ServerJob>();
-        j->t_submit = Clock::now();
-        j->desc.copy_from(*desc);
-        j->dr.assign(h_dr, h_dr + rows);
-        j->dc.assign(h_dc, h_dc + cols);
-        j->v.assign(h_v, h_v + cols * desc->block);
-        j->rounds = rowmix_rounds;
-        if (rowmix_rounds && rows / 2) {
-            require(h_pairs != nullptr, "row-mix plan is required when rounds > 0");
-            j->pairs.assign(h_pairs, h_pairs + (uint64_t)rowmix_rounds * (rows / 2));
-        }
-        j->max_len = max_len;
-        j->rows = rows;
-        j->cols = cols;
-        j->estimate = estimate_job_bytes(*desc, rows, cols, max_len, rowmix_rounds);
-        std::lock_guard<std::mutex> lk(s->mtx);
-        require(!s->stop, "server is shutting down");
-        j->ticket = s->next_ticket++;
-        *ticket = j->ticket;
-        s->queue.push_back(j->ticket);
-        s->jobs.emplace(j->ticket, std::move(j));
-        s->cv_work.notify_one();
-    });
-}
-
-int pgw_server_wait(pgw_server* s, uint64_t ticket, double timeout_s, int* state) {
-    return guarded([&] {
-        require(s && state, "null server or output");
-        std::unique_lock<std::mutex> lk(s->mtx);
-        auto it = s->jobs.find(ticket);
-        require(it != s->jobs.end(), "unknown ticket");
-        ServerJob& j = *it->second;
-        auto pred = [&] { return j.state >= 2; };
-        if (timeout_s < 0) s->cv_done.wait(lk, pred);
-        else s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
-        *state = j.state;
-    });
-}
-
-int pgw_server_result(pgw_server* s, uint64_t ticket, uint32_t* h_seq, uint32_t* h_curv,
-                      uint64_t* len, double* times3) {
-    return guarded([&] {
-        require(s, "null server");
-        std::unique_lock<std::mutex> lk(s->mtx);
-        auto it = s->jobs.find(ticket);
-        require(it != s->jobs.end(), "unknown ticket");
-        ServerJob& j = *it->second;
-        require(j.state >= 2, "job has not finished");
-        if (j.state == 3) throw Error(PGW_ERR_INVALID, "job failed: " + j.error);
-        if (h_seq) std::copy(j.seq.begin(), j.seq.end(), h_seq);
-        if (h_curv) std::copy(j.curv.begin(), j.curv.end(), h_curv);
-        if (len) *len = j.L;
-        if (times3) {
-            times3[0] = std::chrono::duration<double>(j.t_submit - s->t0).count();
-            times3[1] = std::chrono::duration<double>(j.t_admit - s->t0).count();
-            times3[2] = std::chrono::duration<double>(j.t_finish - s->t0).count();
-        }
-        s->jobs.erase(it);
-    });
-}
-
-int pgw_server_stats(pgw_server* s, uint64_t* completed, uint64_t* peak_bytes,
-                     uint64_t* peak_active, uint64_t* queued) {
-    return guarded([&] {
-        require(s != nullptr, "null server");
-        std::lock_guard<std::mutex> lk(s->mtx);
-        if (completed) *completed = s->completed;
-        if (peak_bytes) *peak_bytes = s->peak_bytes;
-        if (peak_active) *peak_active = s->peak_active;
-        if (queued) *queued = s->queue.size();
-    });
-}
-
-int pgw_server_destroy(pgw_server* s) {
-    return guarded([&] {
-        if (!s) return;
-        {
-            std::lock_guard<std::mutex> lk(s->mtx);
-            s->stop = true;
-        }
-        s->cv_work.notify_all();
-        if (s->worker.joinable()) s->worker.join();
-        for (auto& kv : s->jobs)
-            if (kv.second->op) pgw_operator_destroy(kv.second->op);
-        pgw_batch_destroy(s->batch);
-        delete s;
     });
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -41,6 +42,21 @@ void set_last_error(const std::string& msg);
 
 inline void require(bool cond, const std::string& msg, int status = PGW_ERR_INVALID) {
     if (!cond) throw Error(status, msg);
+}
+// literal messages: no string is built unless the check fails (hot loops)
+inline void require(bool cond, const char* msg, int status = PGW_ERR_INVALID) {
+    if (!cond) throw Error(status, msg);
+}
+
+// smallest and largest entry of x[0..n) (one vectorisable pass); {~0u, 0} for n = 0
+inline void min_max_u32(const uint32_t* x, uint64_t n, uint32_t& lo, uint32_t& hi) {
+    uint32_t a = ~0u, b = 0;
+    for (uint64_t q = 0; q < n; ++q) {
+        a = x[q] < a ? x[q] : a;
+        b = x[q] > b ? x[q] : b;
+    }
+    lo = a;
+    hi = b;
 }
 
 // ---------------------------------------------------------------------------
@@ -151,6 +167,11 @@ struct pgw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t* d_binom = nullptr;  // C(a,b), a,b < 64, row-major
+    // Held while a batch captures a graph and around the allocation, upload
+    // and free calls that may run on other threads meanwhile (operator
+    // create/destroy, batch add/result/cancel): such calls during a capture
+    // invalidate it.
+    std::mutex capture_mtx;
 };
 
 namespace pgw {

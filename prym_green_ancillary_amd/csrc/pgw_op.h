@@ -1,0 +1,138 @@
+// pgw_op.h -- internals shared by the C ABI translation units: error
+// guarding, the device-memory arena and the operator object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pgw_kernels.h"
+
+namespace pgw {
+
+// Runs f, turning exceptions into a status code and the thread's last error.
+template <typename F>
+inline int guarded(F&& f) {
+    try {
+        f();
+        return PGW_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return PGW_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return PGW_ERR_INVALID;
+    }
+}
+
+// Device allocations of one operator.  Each HIP call holds `lock` (the
+// context's capture mutex) when one is set, so that another thread's graph
+// capture never sees an allocation, copy or free in the middle.
+struct DeviceArena {
+    std::vector<void*> ptrs;
+    uint64_t bytes = 0;
+    std::mutex* lock = nullptr;
+
+    template <typename T>
+    T* alloc(uint64_t count) {
+        if (count == 0) count = 1;
+        void* p = nullptr;
+        hipError_t e;
+        {
+            std::unique_lock<std::mutex> g = guard();
+            e = hipMalloc(&p, count * sizeof(T));
+        }
+        if (e != hipSuccess)
+            throw Error(PGW_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                            " bytes failed: " + hipGetErrorString(e));
+        ptrs.push_back(p);
+        bytes += count * sizeof(T);
+        return (T*)p;
+    }
+    template <typename T>
+    T* upload(const T* host, uint64_t count) {
+        T* d = alloc<T>(count);
+        if (count) {
+            std::unique_lock<std::mutex> g = guard();
+            PGW_HIP(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+        }
+        return d;
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& v) { return upload(v.data(), (uint64_t)v.size()); }
+    void release() {
+        std::unique_lock<std::mutex> g = guard();
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+        bytes = 0;
+    }
+    std::unique_lock<std::mutex> guard() {
+        return lock ? std::unique_lock<std::mutex>(*lock) : std::unique_lock<std::mutex>();
+    }
+};
+
+}  // namespace pgw
+
+// ---------------------------------------------------------------------------
+
+struct pgw_op {
+    using ModP = pgw::ModP;
+    using DeviceArena = pgw::DeviceArena;
+    using MfmaTable = pgw::MfmaTable;
+    using PhiDevice = pgw::PhiDevice;
+    pgw_ctx* ctx = nullptr;
+    uint32_t kind = 0, p = 0, b = 0, method = PGW_METHOD_PLAIN;
+    ModP m{};
+    uint64_t rows = 0, cols = 0, mid = 0;
+    DeviceArena arena;
+    uint32_t* d_dr = nullptr;
+    uint32_t* d_dc = nullptr;
+    uint32_t* d_negdc = nullptr;
+    pgw_rowmix_pair* d_pairs = nullptr;
+    uint32_t rounds = 0;
+    uint64_t pair_count = 0;
+    int64_t unpaired = -1;
+    uint32_t* d_scaled = nullptr;  // cols x b
+    uint32_t* d_mid = nullptr;     // mid x b
+    uint32_t* d_target = nullptr;  // rows x b
+    uint64_t* d_gram = nullptr;
+    // synthetic
+    uint32_t n = 0, k = 0, M = 0, Mp = 0;
+    uint64_t u_src = 0, u_tgt = 0;
+    uint32_t* d_tab[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t k4[4] = {0, 0, 0, 0};
+    MfmaTable mt[4];
+    int mfma_cfg[4] = {1, 1, 1, 1};  // tile configuration per sweep (pgw_operator_autotune)
+    uint32_t nl = 0;
+    int8_t* d_lp_src = nullptr;
+    int8_t* d_lp_mid = nullptr;
+    // cyclic eliminated
+    PhiDevice dm, d2;
+    uint32_t* d_zoff = nullptr;
+    uint32_t* d_zrank = nullptr;
+    uint16_t* d_zkappa = nullptr;
+    uint32_t* d_R = nullptr;
+    uint32_t* d_Bk = nullptr;
+    uint32_t n_v = 0, kdim = 0;
+    uint64_t x_cols = 0, z_count = 0;
+    uint32_t* d_t = nullptr;
+    uint32_t* d_y = nullptr;
+    uint32_t* d_h = nullptr;
+    uint32_t* d_tadj = nullptr;
+    // timing of the sweep kernels (pgw_bench_apply)
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> sweep_events;
+};
+
+namespace pgw {
+
+// d_out = S d_in on stream st (the block-apply chain, enqueue only).
+void apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st);
+
+}  // namespace pgw
