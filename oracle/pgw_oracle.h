@@ -9,16 +9,22 @@
  * exact: u64/u128 sums and one reduction per output, generalised from the
  * reference's uint16 storage to residues mod any prime p < 2^31.
  *
- * Parity pins (what checks each function, see tests/):
+ * Parity pins (what checks each function, tests/test_oracle.py):
  *   - subsets/incidences: brute force over itertools.combinations order and
- *     the (-1)^position closed form (tests/test_oracle_subsets.py);
+ *     the (-1)^position closed form;
  *   - sweeps / eliminated operator: dense brute-force assembly on tiny inputs,
  *     transpose identities, and the genus-12 certificate K0 (F0*K0 == 0),
  *     F^(0) nnz == 8400 and the sector ranks of g12-reference.json;
  *   - splitmix64 / FNV-1a / mt19937_64: published test vectors;
  *   - uniform_int (libstdc++ 11 Lemire draw): golden vectors written by g++;
  *   - Gram / sequence: closed form V^T S^t V against dense powers;
- *   - WDM text: the hand-made diag(1,2) file of the BCW CLI test.
+ *   - WDM text: the hand-made diag(1,2) file of the BCW CLI test;
+ *   - row-mix plan: structure (disjoint pairs, coefficient range) and the
+ *     transpose identity only.
+ * Parity unpinned: the exact row-mix pair lists / plan hashes and sidecar
+ * hash strings of a real reference run, and the reference binary's own WDM
+ * bytes -- no reference output exists to compare with and the CUDA sources
+ * cannot be built here.
  */
 #ifndef PGW_ORACLE_H
 #define PGW_ORACLE_H
