@@ -8,6 +8,11 @@ resident in HBM before the timed region.  The plain u64 path is timed beside
 it on the same input, and the two outputs must be bit-identical before any
 number is printed.
 
+Other workloads (`--workload`): `large` (n=24, k=6, M=400, M'=440,
+N = 53,838,400, tables from seed 4) the same way; `batch` (64 uneven sectors,
+batched against one at a time); `serve` (a Poisson stream of sector jobs
+through the serving loop, p50/p99 latency).
+
 Prints one JSON line (rank 0).  `--gpus` above 1 exits: this build runs on
 one GPU.
 """
@@ -27,7 +32,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 P31 = 2147483647
-WORKLOAD = dict(n=20, k=5, M=200, M_prime=220, prime=P31, block=16, table_seed=0, seed=2)
+# single-sector workloads: the mid sector (default) and the HBM-scale sector
+WORKLOADS = {
+    "sector": dict(n=20, k=5, M=200, M_prime=220, prime=P31, block=16, table_seed=0, seed=2,
+                   name="single mid sector"),
+    "large": dict(n=24, k=6, M=400, M_prime=440, prime=P31, block=16, table_seed=4, seed=4,
+                  name="large sector at HBM scale"),
+}
 
 
 def macs_per_sweep(n, k, M, Mp, b):
@@ -243,9 +254,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", choices=["sector", "batch", "serve"], default="sector",
-                    help="sector: one mid-size sector (default); batch: 64 uneven sectors; "
-                         "serve: Poisson job stream")
+    ap.add_argument("--workload", choices=["sector", "large", "batch", "serve"], default="sector",
+                    help="sector: one mid-size sector (default); large: the N=53.8M sector; "
+                         "batch: 64 uneven sectors; serve: Poisson job stream")
     ap.add_argument("--rounds", type=int, default=4, help="batch: sequence rounds per sector")
     ap.add_argument("--sectors", type=int, default=64, help="batch: sectors in the mix")
     ap.add_argument("--jobs", type=int, default=24, help="serve: jobs in the stream")
@@ -265,7 +276,7 @@ def main():
     import torch
     import prym_green_ancillary_amd as pg
 
-    w = WORKLOAD
+    w = WORKLOADS[args.workload]
     n, k, M, Mp, p, b = w["n"], w["k"], w["M"], w["M_prime"], w["prime"], w["block"]
     spec = pg.SyntheticSpec(n, k, M, Mp, p, w["table_seed"], w["table_seed"] + 1)
     T, E = spec.tables()
@@ -346,8 +357,9 @@ def main():
         "vs_baseline": None,
         "dtype": "int8-limb/int32 (u32 residues mod 2^31-1)",
         "data": "synthetic",
-        "config": {"workload": "single mid sector c2: n=20 k=5 M=200 M'=220 N=3100800 "
-                               "p=2^31-1 b=16 rowmix 2 rounds", "n": n, "k": k, "M": M,
+        "config": {"workload": f"{w['name']}: n={n} k={k} M={M} M'={Mp} N={N} "
+                               f"p={'2^31-1' if p == P31 else p} b={b} rowmix 2 rounds",
+                   "n": n, "k": k, "M": M,
                    "M_prime": Mp, "N": N, "prime": p, "block": b, "parallelism": "single GPU"},
         "plain_block_applies_per_s": round(plain_rate, 3),
         "mfma_vs_plain": round(value / plain_rate, 3),

@@ -244,3 +244,38 @@ def test_c2_single_sector_apply_mfma_equals_plain_and_oracle(torch_cuda, ctx):
     _log("c2 oracle apply")
     o = oc.synthetic_op(n, k, M, Mp, p, T, E, dr, dc, 2, pairs.reshape(-1))
     assert (o.apply_square(v, b) == sv).all()
+
+
+def test_c4_large_sector_mfma_equals_plain_symmetry_linearity(torch_cuda, ctx):
+    # n=24, k=6, M=400, M'=440 (N = 53,838,400), p = 2^31-1, b = 16, tables from seed 4:
+    # full-size checks through size-independent properties (the oracle would need minutes)
+    n, k, M, Mp, p, b = 24, 6, 400, 440, P31, 16
+    _log("c4 setup")
+    spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=4, seed=4)
+    assert N == 53838400
+    torch = torch_cuda
+    rng = np.random.default_rng(4)
+    y = rng.integers(0, p, N * b, dtype=np.uint32)
+    mf = pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=pg.METHOD_MFMA, tables=(T, E))
+    _log("c4 MFMA applies")
+    sv = gpu_apply(torch, mf, v)
+    sy = gpu_apply(torch, mf, y)
+    vy = ((v.astype(np.uint64) + y) % p).astype(np.uint32)
+    svy = gpu_apply(torch, mf, vy)
+    mf.close()
+    del vy
+    assert (svy == ((sv.astype(np.uint64) + sy) % p).astype(np.uint32)).all()
+    del svy
+
+    def dot(a, c):  # 861M terms below 2^31 each: the int64 sum stays below 2^61
+        acc = 0
+        for s0 in range(0, a.size, 1 << 26):
+            acc += int(((a[s0:s0 + (1 << 26)].astype(np.int64) *
+                         c[s0:s0 + (1 << 26)].astype(np.int64)) % p).sum())
+        return acc % p
+
+    assert dot(sv, y) == dot(v, sy)
+    _log("c4 plain apply")
+    pl = pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=pg.METHOD_PLAIN, tables=(T, E))
+    assert (gpu_apply(torch, pl, v) == sv).all()
+    pl.close()
