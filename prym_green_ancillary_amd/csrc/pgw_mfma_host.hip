@@ -35,7 +35,8 @@ void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, ui
                     host[(((size_t)i * nl + j) * t.rows_pad + r) * t.k_pad + k] =
                         (int8_t)(uint8_t)(((u >> (8 * j)) & 0xff) ^ 0x80);
             }
-    PGW_HIP(hipMalloc(&t.d, host.size()));
+    // + slack: 64-byte chunk loads of a 32-wide tail read up to 32 bytes past the last row
+    PGW_HIP(hipMalloc(&t.d, host.size() + 256));
     PGW_HIP(hipMemcpy(t.d, host.data(), host.size(), hipMemcpyHostToDevice));
 }
 
@@ -110,7 +111,7 @@ void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, ui
 }
 
 // ---------------------------------------------------------------------------
-int mfma_config_count() { return 3; }
+int mfma_config_count() { return 6; }
 
 void launch_dense_mfma_b4(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
                           uint32_t*, const uint32_t*, const ModP&, hipStream_t);
