@@ -60,7 +60,6 @@ DenseSweep sweep_desc(const pgw_op* op, int which) {
     s.r_out = s.fwd ? op->Mp : op->M;
     s.k_in = s.fwd ? op->M : op->Mp;
     s.binom = op->ctx->d_binom;
-    s.zero = op->ctx->d_zero;
     return s;
 }
 
@@ -216,13 +215,9 @@ int pgw_ctx_create(int device, pgw_ctx** out) {
                 for (uint32_t q = 0; q < kk; ++q) v = v * (a - q) / (q + 1);
                 binom[a * 64 + b] = v > (unsigned __int128)UINT64_MAX ? UINT64_MAX : (uint64_t)v;
             }
-        // the table is followed by a 256-byte zero line (source of the MFMA
-        // sweep's loads for absent rows and incidences)
-        binom.resize(64 * 64 + 32, 0);
         e = hipMalloc(&c->d_binom, binom.size() * sizeof(uint64_t));
         if (e == hipSuccess)
             e = hipMemcpy(c->d_binom, binom.data(), binom.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
-        c->d_zero = c->d_binom + 64 * 64;
         if (e != hipSuccess) {
             (void)hipStreamDestroy(c->stream);
             delete c;

@@ -167,7 +167,6 @@ struct pgw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t* d_binom = nullptr;  // C(a,b), a,b < 64, row-major
-    const uint64_t* d_zero = nullptr;  // 256 zero bytes after the table
     // Held while a batch captures a graph and around the allocation, upload
     // and free calls that may run on other threads meanwhile (operator
     // create/destroy, batch add/result/cancel): such calls during a capture

@@ -21,7 +21,6 @@ struct DenseSweep {
     uint32_t r_out = 0;       // output rows per subset panel
     uint32_t k_in = 0;        // input rows per subset panel
     const uint64_t* binom = nullptr;  // device C(a,b) table, 64 x 64 (MFMA path)
-    const void* zero = nullptr;       // device line of >= 16 zero bytes (MFMA path)
 };
 
 // plain u64 path ------------------------------------------------------------

@@ -6,13 +6,24 @@ namespace pgw {
 
 static inline uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 
+// byte of (row, k) in a chunk-major block of `rows` rows and K_pad bytes
+__host__ __device__ inline uint64_t chunk_major_offset(uint64_t row, uint32_t k, uint64_t rows,
+                                                       uint32_t k_pad) {
+    const uint32_t k_full = k_pad / 64 * 64;
+    return k < k_full ? (uint64_t)(k / 64) * rows * 64 + row * 64 + (k % 64)
+                      : (uint64_t)k_full * rows + row * 32 + (k - k_full);
+}
+
 uint32_t mfma_limbs(uint32_t p) { return p <= 65536u ? 2u : 4u; }
 bool mfma_block_supported(uint32_t b) { return b == 4 || b == 8 || b == 16; }
-uint32_t limb_stride(uint32_t k_in) { return round_up(k_in, 16); }
+// K is padded to a multiple of 32 and stored chunk-major: for each 64-byte K
+// chunk the rows (table rows, or panel columns) follow each other, 64 bytes
+// each, and a 32-byte tail chunk (K_pad % 64 == 32) follows with 32 bytes per
+// row.  A 16-row slice of a chunk is one aligned, contiguous 1 KB block.
+uint32_t limb_stride(uint32_t k_in) { return round_up(k_in, 32); }
 
 uint64_t limb_plane_bytes(uint64_t v, uint32_t k_in, uint32_t b, uint32_t nl) {
-    // + slack for fragment reads past the last row (at most K_pad - K_stride bytes)
-    return 2ULL * v * nl * b * limb_stride(k_in) + 4096;
+    return 2ULL * v * nl * b * limb_stride(k_in) + 256;
 }
 
 void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, uint32_t p,
@@ -32,11 +43,11 @@ void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, ui
                                            : src[((size_t)i * rows + r) * k_in + k];
                 uint64_t u = digits_biased(v, p, ds);
                 for (uint32_t j = 0; j < nl; ++j)
-                    host[(((size_t)i * nl + j) * t.rows_pad + r) * t.k_pad + k] =
+                    host[((size_t)i * nl + j) * t.rows_pad * t.k_pad +
+                         chunk_major_offset(r, k, t.rows_pad, t.k_pad)] =
                         (int8_t)(uint8_t)(((u >> (8 * j)) & 0xff) ^ 0x80);
             }
-    // + slack: 64-byte chunk loads of a 32-wide tail read up to 32 bytes past the last row
-    PGW_HIP(hipMalloc(&t.d, host.size() + 256));
+    PGW_HIP(hipMalloc(&t.d, host.size()));
     PGW_HIP(hipMemcpy(t.d, host.data(), host.size(), hipMemcpyHostToDevice));
 }
 
@@ -79,9 +90,10 @@ __global__ __launch_bounds__(256) void k_limb_split(const uint32_t* __restrict__
             w[1][j][tt / 4] |= (uint32_t)((((u1 >> (8 * j)) & 0xff) ^ 0x80) << (8 * (tt % 4)));
         });
     });
+    const uint64_t within = chunk_major_offset((uint64_t)c, kg * 16, B, k_stride);
     static_for<2>([&](auto s) {
         static_for<NL>([&](auto j) {
-            int8_t* dst = lp + ((((uint64_t)s * v_count + v) * NL + j) * B + c) * k_stride + kg * 16;
+            int8_t* dst = lp + (((uint64_t)s * v_count + v) * NL + j) * B * k_stride + within;
             *reinterpret_cast<uint4*>(dst) = make_uint4(w[s][j][0], w[s][j][1], w[s][j][2], w[s][j][3]);
         });
     });
@@ -111,7 +123,7 @@ void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, ui
 }
 
 // ---------------------------------------------------------------------------
-int mfma_config_count() { return 6; }
+int mfma_config_count() { return 3; }
 
 void launch_dense_mfma_b4(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
                           uint32_t*, const uint32_t*, const ModP&, hipStream_t);
