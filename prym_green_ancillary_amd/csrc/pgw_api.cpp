@@ -311,10 +311,10 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
                 require(desc->table_d[q] < op->p && desc->table_e[q] < op->p,
                         "table entries must be reduced modulo p");
             uint32_t method = desc->method;
-            const bool mfma_ok = mfma_block_supported(b) && op->n <= 32 && op->k >= 1;
+            const bool mfma_ok = mfma_block_supported(b) && op->n <= 32 && op->k >= 1 && op->k < 16;
             if (method == PGW_METHOD_AUTO) method = mfma_ok ? PGW_METHOD_MFMA : PGW_METHOD_PLAIN;
             require(method == PGW_METHOD_PLAIN || mfma_ok,
-                    "MFMA path needs b in {4, 8, 16} and n <= 32", PGW_ERR_UNSUPPORTED);
+                    "MFMA path needs b in {4, 8, 16}, n <= 32 and k < 16", PGW_ERR_UNSUPPORTED);
             op->method = method;
             // sweep tables: 0 = D (T, forward), 1 = E (E^T, transpose), 2 = E^T (E, forward),
             // 3 = D^T (T^T, transpose)

@@ -124,19 +124,36 @@ __device__ __forceinline__ bool mfma_block_tile(const MfmaSweepArgs& a, uint64_t
     return true;
 }
 
+// Binomial coefficients C(a, b), a <= 32, b < kBinW, staged in LDS: the
+// subset ranks of the prologue are chains of dependent lookups.
+constexpr int kBinW = 16;
+struct LdsBinom {
+    const uint64_t* t;
+    __device__ uint64_t operator()(int a, int b) const {
+        return (a < 0 || b < 0 || b > a) ? 0ULL : t[a * kBinW + b];
+    }
+};
+
 // Incidence codes of the workgroup's output panels: s_code[g*kMaxN + i] =
 // rank of the neighbour panel | sign << 31, or ~0 when panel g is not incident
-// to table i; s_any[i] = some panel is.  Ends with a barrier.
-template <bool FWD, int SLOTS, int NT>
+// to table i.  s_masks[0] = tables some panel meets (bit i), s_masks[1 + t] =
+// tables the panels of column tile t meet.  Ends with a barrier.
+template <bool FWD, int SLOTS, int SPT, int NT>
 __device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t cg, uint64_t* s_mask,
-                                                uint32_t* s_code, uint32_t* s_any) {
+                                                uint32_t* s_code, uint64_t* s_bin, uint32_t* s_masks) {
+    constexpr int TILES_C = SLOTS / SPT;
     const int tid = threadIdx.x;
-    GlobalBinom C{a.binom};
+    for (int e = tid; e < (kMaxN + 1) * kBinW; e += NT) {
+        const int x = e / kBinW, y = e % kBinW;
+        s_bin[e] = x <= (int)a.n ? a.binom[x * 64 + y] : 0ULL;
+    }
+    if (tid <= TILES_C) s_masks[tid] = 0;
+    __syncthreads();
+    LdsBinom C{s_bin};
     if (tid < SLOTS) {
         const uint64_t u = cg * SLOTS + tid;
         s_mask[tid] = u < a.u_out ? unrank_mask(u, (int)a.n, (int)a.deg_out, C) : 0ULL;
     }
-    if (tid < kMaxN) s_any[tid] = 0;
     __syncthreads();
     for (int idx = tid; idx < SLOTS * (int)a.n; idx += NT) {
         const int g = idx / (int)a.n, i = idx % (int)a.n;
@@ -150,7 +167,8 @@ __device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t
                 const uint64_t nb = FWD ? rank_mask(mask | (1ULL << i), (int)a.n, (int)a.deg_out + 1, C)
                                         : rank_mask(mask & ~(1ULL << i), (int)a.n, (int)a.deg_out - 1, C);
                 code = (uint32_t)nb | (neg << 31);
-                s_any[i] = 1;  // benign race: every writer stores 1
+                atomicOr(&s_masks[0], 1u << i);
+                atomicOr(&s_masks[1 + g / SPT], 1u << i);
             }
         }
         s_code[g * kMaxN + i] = code;
@@ -159,7 +177,19 @@ __device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t
 }
 
 // Epilogue: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile; fold
-// the digit shifts with w_s = 256^s mod p, reduce, scale, store.
+// the digit shifts with w_s = 256^s mod p, reduce, scale, store.  The int32
+// shift sums are first made residues: |acc| < 2^31.
+//  * p = 2^31 - 1: acc + p (if negative) then one conditional subtract; w_s =
+//    2^(8s mod 31), so r * w_s mod p is a 31-bit rotation; the 7 terms sum
+//    below 2^34 and fold once (2^31 == 1).
+//  * 1024 <= p <= 65536 (two digits): acc - floor(acc / p) p from a float
+//    quotient (off by at most one), corrected; r_s * w_s < 2^32, three
+//    terms, one reduction.
+//  * otherwise: 64-bit Barrett per shift.
+__device__ __forceinline__ uint32_t rotl31(uint32_t r, int e) {
+    return e == 0 ? r : (((r << e) | (r >> (31 - e))) & 0x7fffffffu);
+}
+
 template <int B, int NL, int RT, int CT, int WR, int WC>
 __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, uint32_t wg_row0,
                                            v4i (&acc)[RT][CT][2 * NL - 1]) {
@@ -170,21 +200,54 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
     const int wr = wave / WC, wc = wave % WC;
     const int cc = lane % 16;
     const int my_slot = cc / B, my_c = cc % B;
-    const uint64_t off = (uint64_t)a.m.p << 32;  // multiple of p above 2^31
+    const uint32_t p = a.m.p;
+    const bool mersenne = NL == 4 && p == 0x7fffffffu;
+    const float inv_p = 1.0f / (float)p;
+    const bool small_digits = NL == 2 && p >= 1024;  // float quotient exact to +-1
+    auto fold = [&](const int32_t (&x)[NS]) -> uint32_t {
+        if (NL == 4 && mersenne) {
+            uint64_t sum = 0;
+            static_for<NS>([&](auto s) {
+                int64_t t = x[s];
+                uint32_t r = (uint32_t)(t < 0 ? t + (int64_t)p : t);
+                r = r >= p ? r - p : r;
+                sum += rotl31(r, (8 * decltype(s)::value) % 31);
+            });
+            uint32_t v = (uint32_t)((sum & 0x7fffffffu) + (sum >> 31));
+            return v >= p ? v - p : v;
+        } else if (NL == 2 && small_digits) {
+            uint64_t sum = 0;
+            static_for<NS>([&](auto s) {
+                const int32_t t = x[s];
+                const int32_t q = (int32_t)floorf((float)t * inv_p);
+                int64_t r = (int64_t)t - (int64_t)q * p;  // |error| < 2p from the float quotient
+                r = r < 0 ? r + p : r;
+                r = r < 0 ? r + p : r;
+                r = r >= p ? r - p : r;
+                r = r >= p ? r - p : r;
+                sum += (uint64_t)r * a.sw.w[s];
+            });
+            return mod_u64(sum, a.m);
+        } else {
+            const uint64_t off = (uint64_t)p << 32;  // multiple of p above 2^31
+            uint64_t s_lo = 0, s_hi = 0;
+            static_for<NS>([&](auto s) {
+                const uint32_t r = mod_u64((uint64_t)((int64_t)x[s] + (int64_t)off), a.m);
+                const uint64_t prod = (uint64_t)r * a.sw.w[s];  // < 2^62
+                if (s < 4) s_lo += prod; else s_hi += prod;     // 4 terms < 2^64
+            });
+            return addmod(mod_u64(s_lo, a.m), mod_u64(s_hi, a.m), p);
+        }
+    };
     static_for<RT>([&](auto rt) {
         static_for<CT>([&](auto ct) {
             const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
             static_for<4>([&](auto v) {
                 const uint32_t row = wg_row0 + (wr * RT + rt) * 16 + (lane / 16) * 4 + v;
                 if (u >= a.u_out || row >= a.r_out) return;
-                uint64_t s_lo = 0, s_hi = 0;
-                static_for<NS>([&](auto s) {
-                    const int32_t a32 = acc[rt][ct][s][decltype(v)::value];
-                    const uint32_t r = mod_u64((uint64_t)((int64_t)a32 + (int64_t)off), a.m);
-                    const uint64_t prod = (uint64_t)r * a.sw.w[s];  // < 2^62
-                    if (s < 4) s_lo += prod; else s_hi += prod;     // 4 terms < 2^64
-                });
-                uint32_t val = addmod(mod_u64(s_lo, a.m), mod_u64(s_hi, a.m), a.m.p);
+                int32_t x[NS];
+                static_for<NS>([&](auto s) { x[s] = acc[rt][ct][s][decltype(v)::value]; });
+                uint32_t val = fold(x);
                 const uint64_t orow = u * a.r_out + row;
                 if (a.scale) val = mulmod(val, a.scale[orow], a.m);
                 a.out[orow * B + my_c] = val;
@@ -222,7 +285,8 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
     __shared__ __align__(16) int8_t s_b[2][B_BYTES];
     __shared__ uint64_t s_mask[SLOTS];
     __shared__ uint32_t s_code[SLOTS * kMaxN];
-    __shared__ uint32_t s_any[kMaxN];
+    __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
+    __shared__ uint32_t s_masks[1 + TILES_C];
 
     uint64_t cg;
     uint32_t rs;
@@ -230,7 +294,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
     const int tid = threadIdx.x, wave = tid / 64;
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
-    mfma_incidences<FWD, SLOTS, NT>(a, cg, s_mask, s_code, s_any);
+    mfma_incidences<FWD, SLOTS, SPT, NT>(a, cg, s_mask, s_code, s_bin, s_masks);
 
     v4i acc[RT][CT][NS];
     static_for<RT>([&](auto rt) {
@@ -248,20 +312,9 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
     // incidence masks in scalar registers (bit i: table i): tables some panel
     // of the workgroup meets, and those the panels of each of this wave's
     // column tiles meet
-    uint32_t any_mask = 0, on_mask[CT];
-    static_for<CT>([&](auto ct) { on_mask[ct] = 0; });
-    for (int i = 0; i < n; ++i) {
-        any_mask |= (s_any[i] ? 1u : 0u) << i;
-        static_for<CT>([&](auto ct) {
-            bool o = false;
-            static_for<SPT>([&](auto q) {
-                o |= s_code[((wc * CT + ct) * SPT + q) * kMaxN + i] != 0xffffffffu;
-            });
-            on_mask[ct] |= (o ? 1u : 0u) << i;
-        });
-    }
-    any_mask = __builtin_amdgcn_readfirstlane(any_mask);
-    static_for<CT>([&](auto ct) { on_mask[ct] = __builtin_amdgcn_readfirstlane(on_mask[ct]); });
+    const uint32_t any_mask = __builtin_amdgcn_readfirstlane(s_masks[0]);
+    uint32_t on_mask[CT];
+    static_for<CT>([&](auto ct) { on_mask[ct] = __builtin_amdgcn_readfirstlane(s_masks[1 + wc * CT + ct]); });
     auto next_i = [&](int i) {  // first incident table >= i, or n
         const uint32_t rest = i < 32 ? any_mask >> i : 0u;
         return rest ? i + __builtin_ctz(rest) : n;
@@ -447,6 +500,7 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     constexpr int SLOTS = WC * CT * (16 / B);
     constexpr int ROWS = WR * RT * 16;
     require(s.n <= (uint32_t)kMaxN, "MFMA sweep supports n <= 32");
+    require(s.k < (uint32_t)kBinW, "MFMA sweep supports k < 16");
     MfmaSweepArgs a;
     a.binom = s.binom;
     require(a.binom != nullptr, "MFMA sweep needs the device binomial table");

@@ -127,7 +127,11 @@ def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12):
 
 # ----------------------------------------------------------------- synthetic --
 
-@pytest.mark.parametrize("p,b", [(65521, 8), (P31, 16), (P31, 4), (65521, 4), (P31, 8), (65521, 16)])
+# primes chosen for each digit fold of the MFMA epilogue: 2^31-1 (Mersenne
+# rotations), 2147483629 (general 4-digit Barrett), 65521 and 1031 (2-digit
+# float quotient), 1021 and 3 (2-digit Barrett, p < 1024)
+@pytest.mark.parametrize("p,b", [(65521, 8), (P31, 16), (P31, 4), (65521, 4), (P31, 8), (65521, 16),
+                                 (2147483629, 16), (1031, 8), (1021, 16), (3, 4)])
 def test_small_synthetic_plain_and_mfma_vs_oracle(torch_cuda, ctx, p, b):
     n, k, M, Mp = 8, 3, 20, 22
     spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=11)
