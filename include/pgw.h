@@ -34,7 +34,8 @@ enum pgw_status {
     PGW_ERR_HIP = 2,          /* HIP runtime / launch failure */
     PGW_ERR_MEMORY = 3,       /* device or host allocation failed */
     PGW_ERR_UNSUPPORTED = 4,  /* valid request outside what this build runs */
-    PGW_ERR_IO = 5            /* file I/O (WDM writer / reader) */
+    PGW_ERR_IO = 5,           /* file I/O (WDM writer / reader) */
+    PGW_ERR_NEED_MORE_SEQUENCE = 6  /* rank step: no terms beyond those processed */
 };
 
 enum pgw_kind {
@@ -296,6 +297,56 @@ int pgw_server_stats(pgw_server* s, uint64_t* completed, uint64_t* peak_bytes,
                      uint64_t* peak_active, uint64_t* queued);
 /* Finishes every queued and running job, then stops the worker. */
 int pgw_server_destroy(pgw_server* s);
+
+
+/* ---- rank consumer (the reader of the WDM sequence) --------------------- */
+
+/* Replaces the BCW stage `prym-rank` (third_party/bcw/src/rank_pipeline.rs:
+ * 105-287, sigma_basis.rs:37-324): the rank lower bound of the Gram operator
+ * from its symmetric sequence via a shifted order basis of [S(x); I_b] with
+ * shift (0^b, 1^b), and the matrix generator (its first b rows, reversed).
+ * Host code; the same basis as the reference's, for any prime p < 2^31. */
+typedef struct pgw_rank pgw_rank;
+
+/* RankComputationStatus (rank_pipeline.rs:42-47,236-246) */
+enum pgw_rank_status {
+    PGW_RANK_RUNNING = 0,             /* gap below the threshold, more terms may help */
+    PGW_RANK_SUCCESS = 1,             /* gap >= threshold: the rank is final */
+    PGW_RANK_EXCEEDS_DIMENSION = 2,   /* sum of the b smallest degrees > min(rows, cols) */
+    PGW_RANK_SEQUENCE_LIMIT = 3       /* processed >= 2 min(rows, cols) / b + buffer */
+};
+
+/* RankComputationOptions::default() (rank_pipeline.rs:15-16,26-39) */
+#define PGW_RANK_DEFAULT_THRESHOLD 10u
+#define PGW_RANK_DEFAULT_MAX_NLEN_BUFFER 40u
+
+typedef struct {
+    uint64_t rank;          /* rank lower bound: sum of the b smallest shifted degrees */
+    uint64_t delta_spread;  /* gap between the b-th and (b+1)-th smallest degree */
+    uint64_t processed;     /* usable terms consumed (the initial term excluded) */
+    uint64_t max_terms;     /* max_nlen = 2 min(rows, cols) / b + buffer */
+    int status;             /* pgw_rank_status */
+} pgw_rank_info;
+
+/* RankComputation::new (rank_pipeline.rs:119-151).  rows x cols is the size of
+ * F (the WDM header's m n); threads = 0 picks the host's cores (at most 16). */
+int pgw_rank_create(uint32_t prime, uint32_t block, uint64_t n_rows, uint64_t n_cols,
+                    uint32_t threshold, uint32_t max_nlen_buffer, uint32_t threads,
+                    pgw_rank** out);
+/* RankComputation::step (rank_pipeline.rs:153-222): h_seq holds `len` terms of
+ * b(b+1)/2 packed upper-triangle values (term 0 = V^T V, dropped), the layout
+ * of pgw_sequence_run and of the WDM file.  Consumes the terms after those
+ * already processed; PGW_ERR_NEED_MORE_SEQUENCE when there are none. */
+int pgw_rank_step(pgw_rank* r, const uint32_t* h_seq, uint64_t len, pgw_rank_info* info);
+int pgw_rank_get_info(const pgw_rank* r, pgw_rank_info* info);
+/* The 2b shifted degrees, in the basis' row order. */
+int pgw_rank_delta(const pgw_rank* r, uint64_t* h_delta);
+/* order_basis_to_generator_matrixlist + save_generator_list
+ * (sigma_basis.rs:112-168): *terms b x b matrices, matrix k row-major at
+ * h_out[k*b*b], exactly the lines of `<stem>_generators.txt`.  h_out = NULL
+ * only reports *terms. */
+int pgw_rank_generator(const pgw_rank* r, uint32_t* h_out, uint64_t capacity, uint64_t* terms);
+int pgw_rank_destroy(pgw_rank* r);
 
 #ifdef __cplusplus
 }

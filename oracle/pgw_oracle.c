@@ -965,3 +965,176 @@ int orc_write_wdm(const char* path, size_t rows, size_t cols, uint32_t p,
     fclose(f);
     return 0;
 }
+
+/* ---- BCW rank consumer, iterative restatement ----------------------------
+ * third_party/bcw/src/sigma_basis.rs:75-110 (process_input_sequence),
+ * 233-309 (basis), 312-324 (analyze_delta), 112-150 (generator list) and
+ * modular_linalg.rs:71-125 (lsp_decomposition), with the order basis built one
+ * coefficient at a time: at step k the residual coefficient
+ * Delta = [x^k] (M G) is recomputed from scratch, then
+ * M <- (D1 + x Dx) L^-1 pi M exactly as `basis` forms it. */
+
+static void lsp_literal(uint32_t* a, uint32_t* l, size_t m, uint32_t p) {
+    /* a: m x m in/out (becomes S with its columns permuted), l: m x m out */
+    size_t* perm = (size_t*)malloc(m * sizeof(size_t));
+    for (size_t i = 0; i < m; ++i) perm[i] = i;
+    memset(l, 0, m * m * sizeof(uint32_t));
+    for (size_t i = 0; i < m; ++i) l[i * m + i] = 1;
+    size_t rank = 0;
+    for (size_t k = 0; k < m; ++k) {
+        size_t j = m;
+        for (size_t c = rank; c < m; ++c)
+            if (a[k * m + c] % p) { j = c; break; }
+        if (j == m) continue;
+        for (size_t r = 0; r < m; ++r) {
+            uint32_t t = a[r * m + rank];
+            a[r * m + rank] = a[r * m + j];
+            a[r * m + j] = t;
+        }
+        size_t tp = perm[rank]; perm[rank] = perm[j]; perm[j] = tp;
+        uint32_t inv = powmod(a[k * m + rank], p - 2, p);
+        for (size_t i = k + 1; i < m; ++i) {
+            uint32_t factor = mulmod(a[i * m + rank], inv, p);
+            l[i * m + k] = factor;
+            for (size_t col = rank; col < m; ++col) {
+                uint32_t sub = mulmod(factor, a[k * m + col], p);
+                a[i * m + col] = (a[i * m + col] + p - sub) % p;
+            }
+        }
+        ++rank;
+    }
+    free(perm);
+}
+
+/* inverse of a unit lower-triangular matrix by forward substitution */
+static void unit_lower_inverse(const uint32_t* l, uint32_t* inv, size_t m, uint32_t p) {
+    memset(inv, 0, m * m * sizeof(uint32_t));
+    for (size_t c = 0; c < m; ++c) {
+        inv[c * m + c] = 1;
+        for (size_t r = c + 1; r < m; ++r) {
+            uint64_t s = 0;
+            for (size_t t = c; t < r; ++t) s += (uint64_t)l[r * m + t] * inv[t * m + c] % p;
+            inv[r * m + c] = (uint32_t)((p - s % p) % p);
+        }
+    }
+}
+
+int orc_bcw_rank(uint32_t p, size_t b, const uint32_t* seq, size_t L, uint64_t* delta_out,
+                 uint64_t* rank_out, uint64_t* spread_out, uint32_t* gen, size_t gen_cap,
+                 size_t* gen_terms) {
+    if (L < 2 || b == 0) return -1;
+    const size_t d = L - 1, m = 2 * b, n = b, ut = b * (b + 1) / 2, cap = d + 2;
+    /* G[i][j][t], i < 2b, j < b */
+    uint32_t* G = (uint32_t*)calloc(m * n * d, sizeof(uint32_t));
+    size_t ii = 0;
+    for (size_t i = 0; i < b; ++i)
+        for (size_t j = i; j < b; ++j, ++ii)
+            for (size_t t = 0; t < d; ++t) {
+                G[(i * n + j) * d + t] = seq[(t + 1) * ut + ii] % p;
+                G[(j * n + i) * d + t] = seq[(t + 1) * ut + ii] % p;
+            }
+    for (size_t i = 0; i < b; ++i) G[((b + i) * n + i) * d + 0] = 1;
+    /* M[i][j][e], e < cap */
+    uint32_t* M = (uint32_t*)calloc(m * m * cap, sizeof(uint32_t));
+    uint32_t* T = (uint32_t*)calloc(m * m * cap, sizeof(uint32_t));
+    for (size_t i = 0; i < m; ++i) M[(i * m + i) * cap] = 1;
+    uint64_t* delta = (uint64_t*)malloc(m * sizeof(uint64_t));
+    uint64_t* sdelta = (uint64_t*)malloc(m * sizeof(uint64_t));
+    for (size_t i = 0; i < m; ++i) delta[i] = i < b ? 0 : 1;
+    size_t* perm = (size_t*)malloc(m * sizeof(size_t));
+    uint32_t* dm = (uint32_t*)malloc(m * n * sizeof(uint32_t));
+    uint32_t* aug = (uint32_t*)malloc(m * m * sizeof(uint32_t));
+    uint32_t* l = (uint32_t*)malloc(m * m * sizeof(uint32_t));
+    uint32_t* linv = (uint32_t*)malloc(m * m * sizeof(uint32_t));
+    char* shift = (char*)malloc(m);
+    for (size_t k = 0; k < d; ++k) {
+        /* Delta = [x^k] (M G) */
+        for (size_t i = 0; i < m; ++i)
+            for (size_t j = 0; j < n; ++j) {
+                uint64_t s = 0;
+                for (size_t t = 0; t < m; ++t)
+                    for (size_t e = 0; e <= k && e < cap; ++e) {
+                        uint32_t x = M[(i * m + t) * cap + e];
+                        if (x) s += (uint64_t)x * G[(t * n + j) * d + (k - e)] % p;
+                    }
+                dm[i * n + j] = (uint32_t)(s % p);
+            }
+        /* stable sort of the row indices by delta (insertion sort) */
+        for (size_t i = 0; i < m; ++i) perm[i] = i;
+        for (size_t i = 1; i < m; ++i) {
+            size_t x = perm[i], j = i;
+            while (j > 0 && delta[perm[j - 1]] > delta[x]) { perm[j] = perm[j - 1]; --j; }
+            perm[j] = x;
+        }
+        for (size_t i = 0; i < m; ++i) sdelta[i] = delta[perm[i]];
+        /* aug = pi Delta padded to m x m; L S P = aug */
+        memset(aug, 0, m * m * sizeof(uint32_t));
+        for (size_t i = 0; i < m; ++i)
+            for (size_t j = 0; j < n; ++j) aug[i * m + j] = dm[perm[i] * n + j];
+        lsp_literal(aug, l, m, p);
+        unit_lower_inverse(l, linv, m, p);
+        for (size_t i = 0; i < m; ++i) {
+            shift[i] = 0;
+            for (size_t j = 0; j < m; ++j)
+                if (aug[i * m + j]) { shift[i] = 1; break; }
+        }
+        /* T = L^-1 pi M, then rows with a nonzero S row times x */
+        memset(T, 0, m * m * cap * sizeof(uint32_t));
+        for (size_t i = 0; i < m; ++i)
+            for (size_t t = 0; t < m; ++t) {
+                uint32_t f = linv[i * m + t];
+                if (!f) continue;
+                const uint32_t* src = M + perm[t] * m * cap;
+                uint32_t* dst = T + i * m * cap;
+                for (size_t q = 0; q < m * cap; ++q)
+                    if (src[q]) dst[q] = (uint32_t)((dst[q] + (uint64_t)f * src[q]) % p);
+            }
+        for (size_t i = 0; i < m; ++i) {
+            for (size_t j = 0; j < m; ++j) {
+                uint32_t* dst = M + (i * m + j) * cap;
+                const uint32_t* src = T + (i * m + j) * cap;
+                if (shift[i]) {
+                    dst[0] = 0;
+                    memcpy(dst + 1, src, (cap - 1) * sizeof(uint32_t));
+                } else {
+                    memcpy(dst, src, cap * sizeof(uint32_t));
+                }
+            }
+            delta[i] = sdelta[i] + (shift[i] ? 1 : 0);
+        }
+    }
+    /* analyze_delta */
+    for (size_t i = 0; i < m; ++i) sdelta[i] = delta[i];
+    for (size_t i = 1; i < m; ++i) {
+        uint64_t x = sdelta[i];
+        size_t j = i;
+        while (j > 0 && sdelta[j - 1] > x) { sdelta[j] = sdelta[j - 1]; --j; }
+        sdelta[j] = x;
+    }
+    uint64_t rank = 0;
+    for (size_t i = 0; i < b; ++i) rank += sdelta[i];
+    *rank_out = rank;
+    *spread_out = sdelta[b] - sdelta[b - 1];
+    for (size_t i = 0; i < m; ++i) delta_out[i] = delta[i];
+    /* generator: row i < b reversed with its delta, lines after the last
+     * nonzero dropped; line k row-major over (i, j) */
+    size_t maxdeg = 0;
+    for (size_t i = 0; i < b; ++i)
+        for (size_t j = 0; j < b; ++j)
+            for (size_t k = 0; k <= delta[i]; ++k)
+                if (delta[i] - k < cap && M[(i * m + j) * cap + (delta[i] - k)] && k > maxdeg) maxdeg = k;
+    *gen_terms = maxdeg + 1;
+    int rc = 0;
+    if (gen) {
+        if (gen_cap < (maxdeg + 1) * b * b) rc = -2;
+        else
+            for (size_t k = 0; k <= maxdeg; ++k)
+                for (size_t i = 0; i < b; ++i)
+                    for (size_t j = 0; j < b; ++j)
+                        gen[(k * b + i) * b + j] = (k <= delta[i] && delta[i] - k < cap)
+                                                       ? M[(i * m + j) * cap + (delta[i] - k)] : 0;
+    }
+    free(G); free(M); free(T); free(delta); free(sdelta); free(perm); free(dm); free(aug);
+    free(l); free(linv); free(shift);
+    return rc;
+}

@@ -126,6 +126,16 @@ void orc_kernel_vectors(orc_op* op, const uint32_t* v, size_t b, const uint32_t*
 size_t orc_rank_mod_p(const uint32_t* m, size_t rows, size_t cols, uint32_t p,
                       uint32_t* pivot_cols);
 
+/* BCW rank consumer (third_party/bcw/src/sigma_basis.rs:75-150,233-324,
+ * modular_linalg.rs:71-125), one order-basis step per coefficient, residual
+ * recomputed from M G each step.  seq: L x b(b+1)/2 (term 0 = V^T V, dropped).
+ * Outputs the 2b shifted degrees (basis row order), the rank lower bound, the
+ * gap, and the generator matrices (gen[k*b*b + i*b + j], *gen_terms lines).
+ * Returns 0, -1 for L < 2, -2 when gen_cap is too small. */
+int orc_bcw_rank(uint32_t p, size_t b, const uint32_t* seq, size_t L, uint64_t* delta_out,
+                 uint64_t* rank_out, uint64_t* spread_out, uint32_t* gen, size_t gen_cap,
+                 size_t* gen_terms);
+
 /* WDM text writer (cuda/wdmfiles.h:153-222), uncompressed. seq is L x b(b+1)/2. */
 int orc_write_wdm(const char* path, size_t rows, size_t cols, uint32_t p,
                   const uint32_t* dr, const uint32_t* dc, const uint32_t* v,

@@ -22,10 +22,18 @@ __all__ = [
     "reference_random_state", "rowmix_realized_seed", "rowmix_plan", "rowmix_plan_hash",
     "fnv1a64_u32", "synthetic_table", "SyntheticSpec", "synthetic_spec_from_json",
     "load_cyclic_instance", "METHOD_AUTO", "METHOD_PLAIN", "METHOD_MFMA", "sequence_length",
+    "RankComputation", "RANK_RUNNING", "RANK_SUCCESS", "RANK_EXCEEDS_DIMENSION",
+    "RANK_SEQUENCE_LIMIT", "NEED_MORE_SEQUENCE",
 ]
 
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libpgw.so"
+
+# pgw_rank_status (RankComputationStatus) and the need-more-sequence error
+RANK_RUNNING, RANK_SUCCESS, RANK_EXCEEDS_DIMENSION, RANK_SEQUENCE_LIMIT = 0, 1, 2, 3
+NEED_MORE_SEQUENCE = 6
+DEFAULT_STOPPING_THRESHOLD = 10
+DEFAULT_MAX_NLEN_BUFFER = 40
 
 KIND_SYNTHETIC = 1
 KIND_CYCLIC = 2
@@ -63,6 +71,12 @@ class OperatorDesc(ctypes.Structure):
     ]
 
 
+class RankInfo(ctypes.Structure):
+    _fields_ = [("rank", ctypes.c_uint64), ("delta_spread", ctypes.c_uint64),
+                ("processed", ctypes.c_uint64), ("max_terms", ctypes.c_uint64),
+                ("status", ctypes.c_int)]
+
+
 class OpInfo(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_uint64), ("cols", ctypes.c_uint64), ("mid_rows", ctypes.c_uint64),
                 ("device_bytes", ctypes.c_uint64), ("method", ctypes.c_uint32),
@@ -84,7 +98,8 @@ EXPORTED = [
     "pgw_batch_create", "pgw_batch_add", "pgw_batch_step", "pgw_batch_poll", "pgw_batch_result",
     "pgw_batch_cancel", "pgw_batch_graph_builds", "pgw_batch_destroy", "pgw_server_create",
     "pgw_server_submit", "pgw_server_wait", "pgw_server_result", "pgw_server_stats",
-    "pgw_server_destroy", "pgw_job_footprint",
+    "pgw_server_destroy", "pgw_job_footprint", "pgw_rank_create", "pgw_rank_step",
+    "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
 ]
 
 
@@ -149,6 +164,12 @@ def lib():
                                  ctypes.POINTER(u64)]),
         "pgw_server_destroy": (I, [vp]),
         "pgw_job_footprint": (I, [ctypes.POINTER(OperatorDesc), u64, u32, ctypes.POINTER(u64)]),
+        "pgw_rank_create": (I, [u32, u32, u64, u64, u32, u32, u32, ctypes.POINTER(vp)]),
+        "pgw_rank_step": (I, [vp, vp, u64, ctypes.POINTER(RankInfo)]),
+        "pgw_rank_get_info": (I, [vp, ctypes.POINTER(RankInfo)]),
+        "pgw_rank_delta": (I, [vp, vp]),
+        "pgw_rank_generator": (I, [vp, vp, u64, ctypes.POINTER(u64)]),
+        "pgw_rank_destroy": (I, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -549,6 +570,95 @@ class Server:
     def close(self):
         if getattr(self, "h", None):
             lib().pgw_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# rank consumer (third_party/bcw/src/rank_pipeline.rs:105-287)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class RankStep:
+    """RankStep (rank_pipeline.rs:53-60) plus the basis' max_nlen."""
+    processed_sequence_len: int
+    delta_spread: int
+    rank_lower_bound: int
+    max_nlen: int
+    status: int
+
+
+class RankComputation:
+    """RankComputation: shifted order basis of [S(x); I_b] over the usable
+    terms of a symmetric block sequence (term 0 = V^T V is dropped).
+
+    `step(seq)` takes the whole sequence so far, shape (L, b(b+1)/2), and
+    consumes the terms after those already processed; it raises PgwError with
+    status NEED_MORE_SEQUENCE when there are none."""
+
+    def __init__(self, prime: int, block: int, n_rows: int, n_cols: int,
+                 stopping_threshold: int = DEFAULT_STOPPING_THRESHOLD,
+                 max_nlen_buffer: int = DEFAULT_MAX_NLEN_BUFFER, threads: int = 0):
+        self.h = None
+        self.block = block
+        h = ctypes.c_void_p()
+        _check(lib().pgw_rank_create(prime, block, n_rows, n_cols, stopping_threshold,
+                                     max_nlen_buffer, threads, ctypes.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def _step(info: RankInfo) -> RankStep:
+        return RankStep(info.processed, info.delta_spread, info.rank, info.max_terms, info.status)
+
+    def step(self, seq) -> RankStep:
+        ut = self.block * (self.block + 1) // 2
+        seq = _u32(seq).reshape(-1)
+        if seq.size % ut:
+            raise ValueError(f"sequence size {seq.size} is not a multiple of b(b+1)/2 = {ut}")
+        info = RankInfo()
+        _check(lib().pgw_rank_step(self.h, _ptr(seq), seq.size // ut, ctypes.byref(info)))
+        return self._step(info)
+
+    def info(self) -> RankStep:
+        info = RankInfo()
+        _check(lib().pgw_rank_get_info(self.h, ctypes.byref(info)))
+        return self._step(info)
+
+    @property
+    def rank_lower_bound(self) -> int:
+        return self.info().rank_lower_bound
+
+    @property
+    def delta_spread(self) -> int:
+        return self.info().delta_spread
+
+    @property
+    def status(self) -> int:
+        return self.info().status
+
+    def delta(self) -> np.ndarray:
+        out = np.zeros(2 * self.block, np.uint64)
+        _check(lib().pgw_rank_delta(self.h, _ptr(out)))
+        return out
+
+    def generator(self) -> np.ndarray:
+        """The generator matrices (terms, b, b): line k of `<stem>_generators.txt`
+        is matrix k, row-major."""
+        b = self.block
+        terms = ctypes.c_uint64()
+        _check(lib().pgw_rank_generator(self.h, None, 0, ctypes.byref(terms)))
+        out = np.zeros(terms.value * b * b, np.uint32)
+        _check(lib().pgw_rank_generator(self.h, _ptr(out), out.size, ctypes.byref(terms)))
+        return out.reshape(terms.value, b, b)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgw_rank_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -98,6 +98,9 @@ def lib():
     L.orc_write_wdm.restype = _c.c_int
     L.orc_write_wdm.argtypes = [_c.c_char_p, sz, sz, _c.c_uint32, _u32p, _u32p, _u32p, _u32p, sz,
                                 _u32p, sz]
+    L.orc_bcw_rank.restype = _c.c_int
+    L.orc_bcw_rank.argtypes = [_c.c_uint32, sz, _u32p, sz, _c.c_void_p, _c.POINTER(_c.c_uint64),
+                               _c.POINTER(_c.c_uint64), _c.c_void_p, sz, _c.POINTER(sz)]
     _lib = L
     return L
 
@@ -229,6 +232,22 @@ def rank_mod_p(m, p):
     m = u32(m)
     rows, cols = m.shape
     return lib().orc_rank_mod_p(m.reshape(-1), rows, cols, p, None)
+
+
+def bcw_rank(p, b, seq):
+    """Iterative BCW restatement: (delta[2b], rank, spread, generator[terms, b, b])."""
+    seq = np.ascontiguousarray(u32(seq).reshape(-1))
+    L = seq.size // (b * (b + 1) // 2)
+    delta = np.zeros(2 * b, dtype=np.uint64)
+    rank, spread, terms = _c.c_uint64(), _c.c_uint64(), _c.c_size_t()
+    rc = lib().orc_bcw_rank(p, b, seq, L, delta.ctypes.data, _c.byref(rank), _c.byref(spread),
+                            None, 0, _c.byref(terms))
+    assert rc == 0, rc
+    gen = np.zeros(terms.value * b * b, dtype=np.uint32)
+    rc = lib().orc_bcw_rank(p, b, seq, L, delta.ctypes.data, _c.byref(rank), _c.byref(spread),
+                            gen.ctypes.data, gen.size, _c.byref(terms))
+    assert rc == 0, rc
+    return delta, rank.value, spread.value, gen.reshape(terms.value, b, b)
 
 
 def load_json(path):
