@@ -159,9 +159,10 @@ def bench_batch(args):
 
 def bench_serve(args):
     """Poisson stream (seed 5, 2 jobs/s) replayed in real time into the
-    serving loop; each job runs min(2N/b + 8, --max-len) sequence terms.
-    Reports submit-to-finish latency p50/p99 and throughput, against the same
-    stream through a server that runs one job at a time."""
+    serving loop; each job runs min(2N/b + 8, --max-len) sequence terms and,
+    unless --no-rank, the host rank stage on its sequence.  Reports
+    submit-to-finish latency p50/p99 and throughput, against the same stream
+    through a server that runs one job at a time."""
     import prym_green_ancillary_amd as pg
     from prym_green_ancillary_amd import workloads as wl
 
@@ -181,6 +182,8 @@ def bench_serve(args):
 
     def replay(max_active):
         srv = pg.Server(ctx, budget, max_active=max_active, replays_per_step=1)
+        if not args.no_rank:
+            srv.enable_rank(threads=args.rank_threads)
         t0 = time.perf_counter()
         tickets = []
         for j, desc, keep, dr, dc, v, pairs, L in prepared:
@@ -193,7 +196,8 @@ def bench_serve(args):
             if srv.wait(t, 1200.0) != pg.Server.DONE:
                 print("a serving job failed: no number reported", file=sys.stderr)
                 sys.exit(1)
-            res.append(srv.result(t))
+            rk = srv.rank(t)[::2] if not args.no_rank else (None, 0.0)
+            res.append(srv.result(t) + rk)
         wall = time.perf_counter() - t0
         stats = srv.stats()
         srv.close()
@@ -209,7 +213,7 @@ def bench_serve(args):
     mixed, wall, lat, stats = replay(args.max_active)
     alone, wall1, lat1, _ = replay(1)
     for i, (a, b1) in enumerate(zip(mixed, alone)):
-        if not ((a[0] == b1[0]).all() and (a[1] == b1[1]).all()):
+        if not ((a[0] == b1[0]).all() and (a[1] == b1[1]).all()) or a[3] != b1[3]:
             print(f"job {i}: results differ between the mix and one-at-a-time", file=sys.stderr)
             sys.exit(1)
     applies = sum(2 * ((L - 1 + 3) // 4) for (_, _, _, _, _, _, _, L) in prepared)
@@ -239,6 +243,13 @@ def bench_serve(args):
                           "jobs_per_s": round(len(jobs) / wall1, 3)},
         "peak_active": stats["peak_active"], "peak_bytes": stats["peak_bytes"],
         "bit_exact_mix_vs_alone": True,
+        "rank_stage": None if args.no_rank else {
+            "threads_per_job": args.rank_threads,
+            "mean_s": round(float(np.mean([r[4] for r in mixed])), 4),
+            "max_s": round(float(np.max([r[4] for r in mixed])), 4),
+            "status_counts": {str(k): int(v) for k, v in zip(*np.unique(
+                [r[3].status for r in mixed], return_counts=True))},
+        },
         "setup_s": round(t_setup, 2),
         "device": dev_name,
         "roofline": None, "cpu_baseline": None,
@@ -263,6 +274,8 @@ def main():
     ap.add_argument("--max-len", type=int, default=33, help="serve: sequence-term cap per job")
     ap.add_argument("--max-active", type=int, default=16, help="serve: concurrent jobs")
     ap.add_argument("--hbm-fraction", type=float, default=0.5, help="serve: HBM budget share")
+    ap.add_argument("--no-rank", action="store_true", help="serve: skip the host rank stage")
+    ap.add_argument("--rank-threads", type=int, default=2, help="serve: rank-stage threads per job")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); the sector path "

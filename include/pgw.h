@@ -298,7 +298,6 @@ int pgw_server_stats(pgw_server* s, uint64_t* completed, uint64_t* peak_bytes,
 /* Finishes every queued and running job, then stops the worker. */
 int pgw_server_destroy(pgw_server* s);
 
-
 /* ---- rank consumer (the reader of the WDM sequence) --------------------- */
 
 /* Replaces the BCW stage `prym-rank` (third_party/bcw/src/rank_pipeline.rs:
@@ -347,6 +346,21 @@ int pgw_rank_delta(const pgw_rank* r, uint64_t* h_delta);
  * only reports *terms. */
 int pgw_rank_generator(const pgw_rank* r, uint32_t* h_out, uint64_t capacity, uint64_t* terms);
 int pgw_rank_destroy(pgw_rank* r);
+
+/* Host rank stage of a server (no counterpart in the reference, whose
+ * pipeline runs prym-rank on each finished WDM file, scripts/reproduce.py:
+ * 263-266): every job submitted afterwards gets one pgw_rank_step over its
+ * whole sequence, on the server's host workers with `threads` threads, after
+ * its device memory is released.  keep_generator keeps the generator
+ * matrices (terms x b x b values, about rank * b words). */
+int pgw_server_enable_rank(pgw_server* s, uint32_t threads, uint32_t threshold,
+                           uint32_t max_nlen_buffer, int keep_generator);
+/* Rank-stage result of a finished job; call it before pgw_server_result,
+ * which releases the job.  h_gen = NULL skips the generator copy. */
+int pgw_server_rank(pgw_server* s, uint64_t ticket, pgw_rank_info* info, uint32_t* h_gen,
+                    uint64_t gen_capacity, uint64_t* gen_terms, double* seconds);
+
+
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,7 @@ EXPORTED = [
     "pgw_server_submit", "pgw_server_wait", "pgw_server_result", "pgw_server_stats",
     "pgw_server_destroy", "pgw_job_footprint", "pgw_rank_create", "pgw_rank_step",
     "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
+    "pgw_server_enable_rank", "pgw_server_rank",
 ]
 
 
@@ -170,6 +171,9 @@ def lib():
         "pgw_rank_delta": (I, [vp, vp]),
         "pgw_rank_generator": (I, [vp, vp, u64, ctypes.POINTER(u64)]),
         "pgw_rank_destroy": (I, [vp]),
+        "pgw_server_enable_rank": (I, [vp, u32, u32, u32, I]),
+        "pgw_server_rank": (I, [vp, u64, ctypes.POINTER(RankInfo), vp, u64, ctypes.POINTER(u64),
+                                ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -560,6 +564,29 @@ class Server:
         _check(lib().pgw_server_result(self.h, ticket, _ptr(seq), _ptr(curv), ctypes.byref(n),
                                        _ptr(times)))
         return seq.reshape(L, ut), curv, tuple(times.tolist())
+
+    def enable_rank(self, threads: int = 1, stopping_threshold: int = DEFAULT_STOPPING_THRESHOLD,
+                    max_nlen_buffer: int = DEFAULT_MAX_NLEN_BUFFER, keep_generator: bool = False):
+        """Run the rank consumer on every job submitted afterwards (host workers)."""
+        _check(lib().pgw_server_enable_rank(self.h, threads, stopping_threshold, max_nlen_buffer,
+                                            1 if keep_generator else 0))
+        self.keep_generator = keep_generator
+
+    def rank(self, ticket: int):
+        """(RankStep, generator (terms, b, b) or None, seconds) of a finished job;
+        call before result(), which releases the job."""
+        info = RankInfo()
+        terms, secs = ctypes.c_uint64(), ctypes.c_double()
+        _check(lib().pgw_server_rank(self.h, ticket, ctypes.byref(info), None, 0, ctypes.byref(terms),
+                                     ctypes.byref(secs)))
+        gen = None
+        if getattr(self, "keep_generator", False):
+            b = self.meta[ticket][0]
+            gen = np.zeros(terms.value * b * b, np.uint32)
+            _check(lib().pgw_server_rank(self.h, ticket, ctypes.byref(info), _ptr(gen), gen.size,
+                                         ctypes.byref(terms), ctypes.byref(secs)))
+            gen = gen.reshape(terms.value, b, b)
+        return RankComputation._step(info), gen, secs.value
 
     def stats(self):
         c, pb, pa, q = (ctypes.c_uint64() for _ in range(4))

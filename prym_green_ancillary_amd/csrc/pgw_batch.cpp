@@ -437,6 +437,14 @@ struct ServerJob {
     std::vector<uint32_t> seq, curv;
     uint64_t L = 0;
     Clock::time_point t_submit, t_admit, t_started, t_left, t_finish;
+    // host rank stage (pgw_server_enable_rank)
+    bool rank = false, keep_generator = false, ranked = false;
+    int rank_rc = PGW_OK;
+    std::string rank_error;
+    pgw_rank_info rank_info{};
+    std::vector<uint32_t> gen;
+    uint64_t gen_terms = 0;
+    double rank_s = 0;
 };
 
 // Device bytes of a job: operator (tables, digit planes, intermediate blocks,
@@ -479,6 +487,9 @@ struct pgw_server {
     uint64_t next_ticket = 1, in_use = 0, peak_bytes = 0, peak_active = 0, completed = 0;
     uint32_t starting = 0;  // admitted jobs still being set up
     bool stop = false, step_exit = false;
+    bool rank_on = false, rank_keep_gen = false;
+    uint32_t rank_threads = 1, rank_threshold = PGW_RANK_DEFAULT_THRESHOLD,
+             rank_buffer = PGW_RANK_DEFAULT_MAX_NLEN_BUFFER;
     std::vector<std::thread> workers;
     std::thread stepper;
     Clock::time_point t0 = Clock::now();
@@ -592,10 +603,38 @@ struct pgw_server {
         }
         pgw_operator_destroy(j->op);
         j->op = nullptr;
+        {
+            // the device memory is free: admissions need not wait for the rank stage
+            std::lock_guard<std::mutex> lk(mtx);
+            in_use -= j->charge;
+            cv_io.notify_all();
+        }
+        if (rc == PGW_OK && j->rank) rank_job(*j);
         std::lock_guard<std::mutex> lk(mtx);
-        in_use -= j->charge;
         finish(*j, rc == PGW_OK ? DONE : FAILED, msg);
         cv_io.notify_all();
+    }
+
+    // RankComputation over the job's whole sequence (third_party/bcw/src/main.rs:
+    // 46-57 on a completed file), on this worker thread
+    void rank_job(ServerJob& j) {
+        const Clock::time_point t = Clock::now();
+        pgw_rank* r = nullptr;
+        int rc = pgw_rank_create(j.desc.d.prime, j.desc.d.block, j.rows, j.cols, rank_threshold,
+                                 rank_buffer, rank_threads, &r);
+        if (rc == PGW_OK) rc = pgw_rank_step(r, j.seq.data(), j.L, &j.rank_info);
+        if (rc == PGW_OK && j.keep_generator) {
+            rc = pgw_rank_generator(r, nullptr, 0, &j.gen_terms);
+            if (rc == PGW_OK) {
+                j.gen.resize(j.gen_terms * j.desc.d.block * j.desc.d.block);
+                rc = pgw_rank_generator(r, j.gen.data(), j.gen.size(), &j.gen_terms);
+            }
+        }
+        if (rc != PGW_OK) j.rank_error = pgw_last_error();
+        j.rank_rc = rc;
+        pgw_rank_destroy(r);
+        j.ranked = true;
+        j.rank_s = std::chrono::duration<double>(Clock::now() - t).count();
     }
 
     // the GPU thread: batch steps back to back while jobs are running
@@ -690,6 +729,8 @@ int pgw_server_submit(pgw_server* s, const pgw_operator_desc* desc, const uint32
         j->charge = estimate_job_bytes(*desc, rows, cols, max_len, rowmix_rounds);
         std::lock_guard<std::mutex> lk(s->mtx);
         require(!s->stop, "server is shutting down");
+        j->rank = s->rank_on;
+        j->keep_generator = s->rank_keep_gen;
         j->ticket = s->next_ticket++;
         *ticket = j->ticket;
         ServerJob& ref = *j;
@@ -741,6 +782,43 @@ int pgw_server_result(pgw_server* s, uint64_t ticket, uint32_t* h_seq, uint32_t*
             for (int q = 0; q < 5; ++q) times[q] = std::chrono::duration<double>(tp[q] - s->t0).count();
         }
         s->jobs.erase(it);
+    });
+}
+
+int pgw_server_enable_rank(pgw_server* s, uint32_t threads, uint32_t threshold,
+                           uint32_t max_nlen_buffer, int keep_generator) {
+    return guarded([&] {
+        require(s != nullptr, "null server");
+        std::lock_guard<std::mutex> lk(s->mtx);
+        s->rank_on = true;
+        s->rank_threads = threads ? threads : 1;
+        s->rank_threshold = threshold;
+        s->rank_buffer = max_nlen_buffer;
+        s->rank_keep_gen = keep_generator != 0;
+    });
+}
+
+int pgw_server_rank(pgw_server* s, uint64_t ticket, pgw_rank_info* info, uint32_t* h_gen,
+                    uint64_t gen_capacity, uint64_t* gen_terms, double* seconds) {
+    return guarded([&] {
+        require(s != nullptr, "null server");
+        std::lock_guard<std::mutex> lk(s->mtx);
+        auto it = s->jobs.find(ticket);
+        require(it != s->jobs.end(), "unknown ticket");
+        ServerJob& j = *it->second;
+        require(j.state >= DONE, "job has not finished");
+        require(j.state == DONE, "job failed: " + j.error);
+        require(j.ranked, "the job was submitted without the rank stage");
+        if (j.rank_rc != PGW_OK) throw Error(j.rank_rc, "rank stage: " + j.rank_error);
+        if (info) *info = j.rank_info;
+        if (seconds) *seconds = j.rank_s;
+        if (gen_terms) *gen_terms = j.gen_terms;
+        if (h_gen) {
+            require(j.keep_generator, "the generator was not kept (keep_generator = 0)");
+            require(gen_capacity >= j.gen.size(), "generator buffer too small: " +
+                                                      std::to_string(j.gen.size()) + " values needed");
+            std::copy(j.gen.begin(), j.gen.end(), h_gen);
+        }
     });
 }
 
