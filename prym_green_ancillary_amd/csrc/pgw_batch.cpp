@@ -463,8 +463,8 @@ uint64_t estimate_job_bytes(const pgw_operator_desc& d, uint64_t rows, uint64_t 
         const uint64_t rp = (std::max(d.m_src, d.m_tgt) + 15) / 16 * 16 + 16;
         const uint64_t kp = (std::max(d.m_src, d.m_tgt) + 31) / 32 * 32 + 32;
         bytes += 4 * (uint64_t)d.n * 4 * rp * kp * 2;          // tables (either path, upper bound)
-        bytes += 2 * us * 4 * b * (((uint64_t)d.m_src + 15) / 16 * 16);   // source digit planes
-        bytes += 2 * ut2 * 4 * b * (((uint64_t)d.m_tgt + 15) / 16 * 16);  // mid digit planes
+        bytes += 2 * us * 4 * b * (((uint64_t)d.m_src + 31) / 32 * 32);   // source digit planes
+        bytes += 2 * ut2 * 4 * b * (((uint64_t)d.m_tgt + 31) / 32 * 32);  // mid digit planes
     } else {
         bytes += 6 * (rows + cols) * b * 4 + 64ull * (rows + cols);
     }

@@ -5,10 +5,13 @@
 
 namespace pgw {
 
+constexpr size_t kMfmaZeroBytes = 1024;  // zero block after the table digits
+
 // Table-stationary digit planes of a synthetic multiplication table for the
 // MFMA sweep: tabl[i][limb][R_pad][K_pad] int8 (balanced base-256 digits).
 struct MfmaTable {
     int8_t* d = nullptr;
+    const int8_t* zero = nullptr;  // 1 KB of zero bytes after the digits (inside the allocation of d)
     uint32_t rows = 0, k_in = 0, rows_pad = 0, k_pad = 0;
 };
 

@@ -47,8 +47,11 @@ void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, ui
                          chunk_major_offset(r, k, t.rows_pad, t.k_pad)] =
                         (int8_t)(uint8_t)(((u >> (8 * j)) & 0xff) ^ 0x80);
             }
+    const size_t digits = host.size();
+    host.resize(digits + kMfmaZeroBytes, 0);
     PGW_HIP(hipMalloc(&t.d, host.size()));
     PGW_HIP(hipMemcpy(t.d, host.data(), host.size(), hipMemcpyHostToDevice));
+    t.zero = t.d + digits;
 }
 
 // ---------------------------------------------------------------------------
@@ -123,7 +126,7 @@ void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, ui
 }
 
 // ---------------------------------------------------------------------------
-int mfma_config_count() { return 3; }
+int mfma_config_count() { return 6; }
 
 void launch_dense_mfma_b4(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
                           uint32_t*, const uint32_t*, const ModP&, hipStream_t);

@@ -40,8 +40,6 @@ namespace pgw {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-
 template <typename F, int... I>
 __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
     (f(std::integral_constant<int, I>{}), ...);
@@ -83,6 +81,7 @@ struct MfmaSweepArgs {
     const uint64_t* binom;  // C(a,b) table, 64 x 64
     const int8_t* tabl;
     const int8_t* lp;
+    const int8_t* zero;     // 1 KB of zero bytes (staging loads that have no data)
     uint32_t n, deg_out;
     uint64_t u_out, v_in;
     uint32_t r_out, r_pad, k_in, k_pad;
@@ -493,8 +492,256 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
     mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
 }
 
+// ---------------------------------------------------------------------------
+// direct-to-LDS variant: three LDS stages fed by global_load_lds
+//
+// Same tiles and MFMA schedule as above, but the staging copies go straight
+// from global memory into LDS (global_load_lds_dwordx4: every lane moves 16
+// bytes, a wave fills one contiguous 1 KB block), so they need no VGPRs and
+// no LDS store instructions, and three stages keep two chunks of loads in
+// flight: at chunk c the loads of chunk c+2 are issued into the stage chunk
+// c-1 used, while the MFMAs of chunk c run.  Each wave waits (vmcnt) for its
+// own loads of chunk c, then the workgroup barrier makes every wave's loads
+// visible.  The 16-byte piece a lane moves is picked so that the block lands
+// in the swizzled layout of lds_off().  Loads that have no data (rows past
+// the table, panels not incident to the table, the missing partner of a
+// tail pair) read a 1 KB zero block.
+// ---------------------------------------------------------------------------
+
+// workgroup barrier that orders LDS accesses only (global loads stay in flight)
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One global_load_lds_dwordx4: lane L copies 16 bytes from src (its own
+// address) to LDS byte lds + 16 L.  Written as inline asm because the
+// compiler, which cannot tell which LDS buffer such a load writes, would
+// otherwise wait for every one in flight before any LDS read; the kernel
+// waits for them itself (wait_vmcnt before its barrier).  The instruction
+// returns nothing to VGPRs, so no output register is left pending.
+__device__ __forceinline__ void load_to_lds(const void* src, int8_t* lds) {
+    const uint32_t m0 = __builtin_amdgcn_readfirstlane(
+        (uint32_t)(uintptr_t)(__attribute__((address_space(3))) int8_t*)lds);
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(m0)
+                 : "memory", "m0");
+}
+
+// s_waitcnt with only the vector-memory counter constrained (at most n loads
+// of this wave still in flight); n is a compile-time constant below 64
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+template <int B, int NL, int RT, int CT, int WR, int WC, bool FWD>
+__global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mfma_dma(MfmaSweepArgs a) {
+    constexpr int NW = WR * WC;
+    constexpr int NT = 64 * NW;
+    constexpr int NS = 2 * NL - 1;
+    constexpr int SPT = 16 / B;
+    constexpr int TILES_C = WC * CT;
+    constexpr int SLOTS = TILES_C * SPT;
+    constexpr int ROWS = WR * RT * 16;
+    constexpr int A_BYTES = NL * ROWS * kLdsStride;
+    constexpr int B_BYTES = TILES_C * NL * 16 * kLdsStride;
+    constexpr int A_BLOCKS = A_BYTES / 1024, B_BLOCKS = B_BYTES / 1024;  // 1 KB blocks per stage
+    static_assert((A_BLOCKS + B_BLOCKS) % NW == 0, "blocks must split evenly over the waves");
+    constexpr int PER = (A_BLOCKS + B_BLOCKS) / NW;  // loads per wave and chunk
+
+    // three arrays (not one indexed array) so that the compiler can tell the
+    // stage being read from the stages still being loaded
+    __shared__ __align__(1024) int8_t s_st0[A_BYTES + B_BYTES];
+    __shared__ __align__(1024) int8_t s_st1[A_BYTES + B_BYTES];
+    __shared__ __align__(1024) int8_t s_st2[A_BYTES + B_BYTES];
+    __shared__ uint64_t s_mask[SLOTS];
+    __shared__ uint32_t s_code[SLOTS * kMaxN];
+    __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
+    __shared__ uint32_t s_masks[1 + TILES_C];
+
+    uint64_t cg;
+    uint32_t rs;
+    if (!mfma_block_tile(a, cg, rs)) return;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int wr = wave / WC, wc = wave % WC;
+    const uint32_t wg_row0 = rs * ROWS;
+    mfma_incidences<FWD, SLOTS, SPT, NT>(a, cg, s_mask, s_code, s_bin, s_masks);
+
+    v4i acc[RT][CT][NS];
+    static_for<RT>([&](auto rt) {
+        static_for<CT>([&](auto ct) { static_for<NS>([&](auto s) { acc[rt][ct][s] = v4i{0, 0, 0, 0}; }); });
+    });
+
+    const uint32_t k_pad = a.k_pad;
+    const uint64_t plane = (uint64_t)NL * B * k_pad;
+    const uint32_t limb_block_b = B * k_pad;
+    const uint32_t limb_block_a = a.r_pad * k_pad;
+    const uint32_t table_stride = NL * limb_block_a;
+    const int n = (int)a.n;
+    const uint32_t k_full = k_pad / 64 * 64;
+    const bool has_tail = k_full < k_pad;
+    const uint32_t any_mask = __builtin_amdgcn_readfirstlane(s_masks[0]);
+    uint32_t on_mask[CT];
+    static_for<CT>([&](auto ct) { on_mask[ct] = __builtin_amdgcn_readfirstlane(s_masks[1 + wc * CT + ct]); });
+    auto next_i = [&](int i) {
+        const uint32_t rest = i < 32 ? any_mask >> i : 0u;
+        return rest ? i + __builtin_ctz(rest) : n;
+    };
+    auto chunk_on = [&](uint32_t mask, int i, int i2, uint32_t k0) {
+        const uint32_t both = (1u << i) | ((k0 == k_full && i2 < n) ? (1u << i2) : 0u);
+        return (mask & both) != 0;
+    };
+
+    // lane L of a load fills bytes [16L, 16L+16) of a 1 KB block: staged row
+    // L/4 of the block, physical piece L%4, i.e. logical piece (L%4 - row/4) & 3
+    const int8_t* zero = a.zero + lane * 16;
+    const uint32_t zero_off = (uint32_t)(zero - a.tabl);
+    auto stage = [&](auto st) -> int8_t* {
+        constexpr int S = decltype(st)::value % 3;
+        if constexpr (S == 0) return s_st0;
+        else if constexpr (S == 1) return s_st1;
+        else return s_st2;
+    };
+    auto issue = [&](int8_t* dst, int i, int i2, uint32_t k0) {
+        const bool tail = k0 == k_full;
+        const uint32_t wshift = tail ? 5u : 6u;  // bytes per chunk row: 32 in the tail, else 64
+        static_for<PER>([&](auto q) {
+            const int blk = wave + (int)decltype(q)::value * NW;  // wave-uniform
+            const int row_s = blk * 16 + lane / 4;               // staged row of the stage
+            const int lpc = ((lane % 4) - (row_s >> 2)) & 3;     // logical 16-byte piece
+            const int t = (tail && lpc >= 2) ? i2 : i;
+            const uint32_t poff = tail ? (lpc & 1) * 16 : lpc * 16;
+            const int8_t* src;
+            if (blk < A_BLOCKS) {  // table rows: staged row = j * ROWS + r
+                const int r = row_s % ROWS, j = row_s / ROWS;
+                const uint32_t row = wg_row0 + r;
+                const uint32_t off = (t < n && row < a.r_pad)
+                                         ? (uint32_t)t * table_stride + k0 * a.r_pad + j * limb_block_a +
+                                               (row << wshift) + poff
+                                         : zero_off;
+                src = a.tabl + off;
+            } else {  // panel columns: staged row - A rows = (tile * NL + l) * 16 + c16
+                const int rb = row_s - A_BLOCKS * 16;
+                const int c16 = rb % 16, l = rb / 16 % NL, tile = rb / 16 / NL;
+                const uint32_t cd = t < n ? s_code[(tile * SPT + c16 / B) * kMaxN + t] : 0xffffffffu;
+                src = cd != 0xffffffffu
+                          ? a.lp + ((uint64_t)(cd >> 31) * a.v_in + (cd & 0x7fffffffu)) * plane +
+                                ((uint64_t)k0 * B + l * limb_block_b + ((uint32_t)(c16 % B) << wshift) + poff)
+                          : zero;
+            }
+            load_to_lds(src, dst + blk * 1024);
+        });
+    };
+
+    auto first = [&](int& i, int& i2, uint32_t& k0) {
+        i = next_i(0);
+        if (k_full > 0) {
+            i2 = n;
+            k0 = 0;
+        } else {
+            i2 = i < n ? next_i(i + 1) : n;
+            k0 = k_full;
+        }
+    };
+    auto advance = [&](int& i, int& i2, uint32_t& k0) {
+        if (k0 != k_full) {
+            k0 += 64;
+            if (k0 < k_full) return;
+            i = next_i(i + 1);
+            if (i < n) {
+                k0 = 0;
+                return;
+            }
+            if (!has_tail) return;
+            i = next_i(0);
+            i2 = i < n ? next_i(i + 1) : n;
+            k0 = k_full;
+            return;
+        }
+        i = i2 < n ? next_i(i2 + 1) : n;
+        i2 = i < n ? next_i(i + 1) : n;
+    };
+
+    bool row_on[RT];
+    static_for<RT>([&](auto rt) {
+        row_on[rt] = __builtin_amdgcn_readfirstlane((int)(wg_row0 + (wr * RT + rt) * 16 < a.r_pad)) != 0;
+    });
+
+    const int cc = lane % 16, kq = (lane / 16) * 16;
+    // chunk c = (i, i2, k0) in stage ST (c mod 3); c+1 = (j1, j12, q1); false after the last chunk
+    int i, i2, j1, j12;
+    uint32_t k0, q1;
+    auto step = [&](auto ST) {
+        constexpr int S = decltype(ST)::value;
+        // this wave's loads of chunk c are done (chunk c+1's may stay in flight);
+        // after the barrier every wave's are, and stage c-1 is free
+        if (j1 < n) wait_vmcnt<PER>(); else wait_vmcnt<0>();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        int j2 = j1, j22 = j12;
+        uint32_t q2 = q1;
+        if (j1 < n) {
+            advance(j2, j22, q2);
+            if (j2 < n) issue(stage(std::integral_constant<int, S + 2>{}), j2, j22, q2);
+        }
+        const int8_t* sa = stage(ST);
+        const int8_t* sb = sa + A_BYTES;
+        v4i fa[RT][NL], fb[CT][NL];
+        static_for<CT>([&](auto ct) {
+            static_for<NL>([&](auto l) {
+                fb[ct][l] = *reinterpret_cast<const v4i*>(sb + ((wc * CT + ct) * NL + l) * 16 * kLdsStride +
+                                                          lds_off(cc, kq));
+            });
+            if constexpr (decltype(ct)::value == 0) {
+                static_for<RT>([&](auto rt) {
+                    static_for<NL>([&](auto j) {
+                        fa[rt][j] = *reinterpret_cast<const v4i*>(
+                            sa + (j * ROWS + wr * RT * 16 + rt * 16) * kLdsStride + lds_off(cc, kq));
+                    });
+                });
+            }
+        });
+        static_for<CT>([&](auto ct) {
+            if (!chunk_on(on_mask[ct], i, i2, k0)) return;
+            static_for<RT>([&](auto rt) {
+                if (!row_on[rt]) return;
+                static_for<NL>([&](auto j) {
+                    static_for<NL>([&](auto l) {
+                        acc[rt][ct][j + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[rt][j], fb[ct][l],
+                                                                                  acc[rt][ct][j + l], 0, 0, 0);
+                    });
+                });
+            });
+        });
+        const bool more = j1 < n;
+        i = j1;
+        i2 = j12;
+        k0 = q1;
+        j1 = j2;
+        j12 = j22;
+        q1 = q2;
+        return more;
+    };
+    first(i, i2, k0);
+    if (i < n) {
+        j1 = i;
+        j12 = i2;
+        q1 = k0;
+        advance(j1, j12, q1);
+        issue(s_st0, i, i2, k0);
+        if (j1 < n) issue(s_st1, j1, j12, q1);
+        while (step(std::integral_constant<int, 0>{}) && step(std::integral_constant<int, 1>{}) &&
+               step(std::integral_constant<int, 2>{})) {
+        }
+    }
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
+}
+
 // kernel variants of a tile configuration
-template <int B, int NL, int RT, int CT, int WR, int WC>
+template <int B, int NL, int RT, int CT, int WR, int WC, bool DMA = false>
 static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
                            const uint32_t* scale, const ModP& m, hipStream_t st) {
     constexpr int SLOTS = WC * CT * (16 / B);
@@ -506,6 +753,7 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     require(a.binom != nullptr, "MFMA sweep needs the device binomial table");
     a.tabl = t.d;
     a.lp = lp;
+    a.zero = t.zero;
     a.n = s.n;
     a.deg_out = s.fwd ? s.k - 1 : s.k;
     a.u_out = s.u_out;
@@ -530,10 +778,13 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     const uint64_t blocks = a.xcd_affinity ? 8 * ((a.col_groups + G - 1) / G)
                                            : a.col_groups * a.row_slices;
     require(blocks < (1ULL << 31), "MFMA sweep grid too large");
-#define PGW_MF(F)                                                                                 \
-    hipLaunchKernelGGL((k_dense_mfma<B, NL, RT, CT, WR, WC, F>), dim3((unsigned)blocks),          \
-                       dim3(64 * WR * WC), 0, st, a)
-    if (s.fwd) PGW_MF(true); else PGW_MF(false);
+#define PGW_MF(K, F) hipLaunchKernelGGL((K<B, NL, RT, CT, WR, WC, F>), dim3((unsigned)blocks), \
+                                        dim3(64 * WR * WC), 0, st, a)
+    if constexpr (DMA) {
+        if (s.fwd) PGW_MF(k_dense_mfma_dma, true); else PGW_MF(k_dense_mfma_dma, false);
+    } else {
+        if (s.fwd) PGW_MF(k_dense_mfma, true); else PGW_MF(k_dense_mfma, false);
+    }
 #undef PGW_MF
     PGW_CHECK_LAUNCH("MFMA sweep");
 }
@@ -545,13 +796,19 @@ static void dense_mfma_pick(int cfg, const DenseSweep& s, const MfmaTable& t, co
         switch (cfg) {
             case 0: dense_mfma_cfg<B, NL, 2, 2, 2, 2>(s, t, lp, out, scale, m, st); return;
             case 1: dense_mfma_cfg<B, NL, 2, 2, 4, 2>(s, t, lp, out, scale, m, st); return;
-            default: dense_mfma_cfg<B, NL, 2, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
+            case 2: dense_mfma_cfg<B, NL, 2, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
+            case 3: dense_mfma_cfg<B, NL, 2, 2, 4, 2, true>(s, t, lp, out, scale, m, st); return;
+            case 4: dense_mfma_cfg<B, NL, 2, 2, 2, 2, true>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 2, 2, 2, 4, true>(s, t, lp, out, scale, m, st); return;
         }
     } else {
         switch (cfg) {
             case 0: dense_mfma_cfg<B, NL, 2, 4, 2, 2>(s, t, lp, out, scale, m, st); return;
             case 1: dense_mfma_cfg<B, NL, 2, 4, 4, 2>(s, t, lp, out, scale, m, st); return;
-            default: dense_mfma_cfg<B, NL, 4, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
+            case 2: dense_mfma_cfg<B, NL, 4, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
+            case 3: dense_mfma_cfg<B, NL, 2, 4, 4, 2, true>(s, t, lp, out, scale, m, st); return;
+            case 4: dense_mfma_cfg<B, NL, 2, 4, 2, 2, true>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 4, 2, 2, 4, true>(s, t, lp, out, scale, m, st); return;
         }
     }
 }
