@@ -536,7 +536,7 @@ __device__ __forceinline__ void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
-template <int B, int NL, int RT, int CT, int WR, int WC, bool FWD>
+template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
 __global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mfma_dma(MfmaSweepArgs a) {
     constexpr int NW = WR * WC;
     constexpr int NT = 64 * NW;
@@ -549,13 +549,14 @@ __global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mf
     constexpr int B_BYTES = TILES_C * NL * 16 * kLdsStride;
     constexpr int A_BLOCKS = A_BYTES / 1024, B_BLOCKS = B_BYTES / 1024;  // 1 KB blocks per stage
     static_assert((A_BLOCKS + B_BLOCKS) % NW == 0, "blocks must split evenly over the waves");
+    static_assert(STG == 2 || STG == 3, "two or three LDS stages");
     constexpr int PER = (A_BLOCKS + B_BLOCKS) / NW;  // loads per wave and chunk
 
-    // three arrays (not one indexed array) so that the compiler can tell the
-    // stage being read from the stages still being loaded
+    // one array per stage (not one indexed array), stage indices known at
+    // compile time: the stage read and the stages being loaded stay apart
     __shared__ __align__(1024) int8_t s_st0[A_BYTES + B_BYTES];
     __shared__ __align__(1024) int8_t s_st1[A_BYTES + B_BYTES];
-    __shared__ __align__(1024) int8_t s_st2[A_BYTES + B_BYTES];
+    __shared__ __align__(1024) int8_t s_st2[STG == 3 ? A_BYTES + B_BYTES : 16];
     __shared__ uint64_t s_mask[SLOTS];
     __shared__ uint32_t s_code[SLOTS * kMaxN];
     __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
@@ -599,7 +600,7 @@ __global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mf
     const int8_t* zero = a.zero + lane * 16;
     const uint32_t zero_off = (uint32_t)(zero - a.tabl);
     auto stage = [&](auto st) -> int8_t* {
-        constexpr int S = decltype(st)::value % 3;
+        constexpr int S = decltype(st)::value % STG;
         if constexpr (S == 0) return s_st0;
         else if constexpr (S == 1) return s_st1;
         else return s_st2;
@@ -675,17 +676,25 @@ __global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mf
     uint32_t k0, q1;
     auto step = [&](auto ST) {
         constexpr int S = decltype(ST)::value;
-        // this wave's loads of chunk c are done (chunk c+1's may stay in flight);
-        // after the barrier every wave's are, and stage c-1 is free
-        if (j1 < n) wait_vmcnt<PER>(); else wait_vmcnt<0>();
+        // this wave's loads of chunk c are done (with three stages chunk
+        // c+1's may stay in flight); after the barrier every wave's are, and
+        // the stage of chunk c-1 is free
+        if (STG == 3 && j1 < n) wait_vmcnt<PER>(); else wait_vmcnt<0>();
         asm volatile("" ::: "memory");
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
         int j2 = j1, j22 = j12;
         uint32_t q2 = q1;
-        if (j1 < n) {
-            advance(j2, j22, q2);
-            if (j2 < n) issue(stage(std::integral_constant<int, S + 2>{}), j2, j22, q2);
+        if constexpr (STG == 3) {
+            if (j1 < n) {
+                advance(j2, j22, q2);
+                if (j2 < n) issue(stage(std::integral_constant<int, S + 2>{}), j2, j22, q2);
+            }
+        } else {
+            if (j1 < n) {
+                issue(stage(std::integral_constant<int, S + 1>{}), j1, j12, q1);
+                advance(j2, j22, q2);
+            }
         }
         const int8_t* sa = stage(ST);
         const int8_t* sb = sa + A_BYTES;
@@ -732,16 +741,21 @@ __global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mf
         q1 = k0;
         advance(j1, j12, q1);
         issue(s_st0, i, i2, k0);
-        if (j1 < n) issue(s_st1, j1, j12, q1);
-        while (step(std::integral_constant<int, 0>{}) && step(std::integral_constant<int, 1>{}) &&
-               step(std::integral_constant<int, 2>{})) {
+        if constexpr (STG == 3) {
+            if (j1 < n) issue(s_st1, j1, j12, q1);
+            while (step(std::integral_constant<int, 0>{}) && step(std::integral_constant<int, 1>{}) &&
+                   step(std::integral_constant<int, 2>{})) {
+            }
+        } else {
+            while (step(std::integral_constant<int, 0>{}) && step(std::integral_constant<int, 1>{})) {
+            }
         }
     }
     mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
 }
 
 // kernel variants of a tile configuration
-template <int B, int NL, int RT, int CT, int WR, int WC, bool DMA = false>
+template <int B, int NL, int RT, int CT, int WR, int WC, int DMA = 0>
 static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
                            const uint32_t* scale, const ModP& m, hipStream_t st) {
     constexpr int SLOTS = WC * CT * (16 / B);
@@ -778,10 +792,10 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     const uint64_t blocks = a.xcd_affinity ? 8 * ((a.col_groups + G - 1) / G)
                                            : a.col_groups * a.row_slices;
     require(blocks < (1ULL << 31), "MFMA sweep grid too large");
-#define PGW_MF(K, F) hipLaunchKernelGGL((K<B, NL, RT, CT, WR, WC, F>), dim3((unsigned)blocks), \
-                                        dim3(64 * WR * WC), 0, st, a)
+#define PGW_MF(K, ...) hipLaunchKernelGGL((K<B, NL, RT, CT, WR, WC, __VA_ARGS__>), dim3((unsigned)blocks), \
+                                          dim3(64 * WR * WC), 0, st, a)
     if constexpr (DMA) {
-        if (s.fwd) PGW_MF(k_dense_mfma_dma, true); else PGW_MF(k_dense_mfma_dma, false);
+        if (s.fwd) PGW_MF(k_dense_mfma_dma, DMA, true); else PGW_MF(k_dense_mfma_dma, DMA, false);
     } else {
         if (s.fwd) PGW_MF(k_dense_mfma, true); else PGW_MF(k_dense_mfma, false);
     }
@@ -797,18 +811,18 @@ static void dense_mfma_pick(int cfg, const DenseSweep& s, const MfmaTable& t, co
             case 0: dense_mfma_cfg<B, NL, 2, 2, 2, 2>(s, t, lp, out, scale, m, st); return;
             case 1: dense_mfma_cfg<B, NL, 2, 2, 4, 2>(s, t, lp, out, scale, m, st); return;
             case 2: dense_mfma_cfg<B, NL, 2, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
-            case 3: dense_mfma_cfg<B, NL, 2, 2, 4, 2, true>(s, t, lp, out, scale, m, st); return;
-            case 4: dense_mfma_cfg<B, NL, 2, 2, 2, 2, true>(s, t, lp, out, scale, m, st); return;
-            default: dense_mfma_cfg<B, NL, 2, 2, 2, 4, true>(s, t, lp, out, scale, m, st); return;
+            case 3: dense_mfma_cfg<B, NL, 2, 2, 4, 2, 3>(s, t, lp, out, scale, m, st); return;
+            case 4: dense_mfma_cfg<B, NL, 2, 2, 8, 1, 2>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 2, 2, 2, 4, 3>(s, t, lp, out, scale, m, st); return;
         }
     } else {
         switch (cfg) {
             case 0: dense_mfma_cfg<B, NL, 2, 4, 2, 2>(s, t, lp, out, scale, m, st); return;
             case 1: dense_mfma_cfg<B, NL, 2, 4, 4, 2>(s, t, lp, out, scale, m, st); return;
             case 2: dense_mfma_cfg<B, NL, 4, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
-            case 3: dense_mfma_cfg<B, NL, 2, 4, 4, 2, true>(s, t, lp, out, scale, m, st); return;
-            case 4: dense_mfma_cfg<B, NL, 2, 4, 2, 2, true>(s, t, lp, out, scale, m, st); return;
-            default: dense_mfma_cfg<B, NL, 4, 2, 2, 4, true>(s, t, lp, out, scale, m, st); return;
+            case 3: dense_mfma_cfg<B, NL, 2, 4, 4, 2, 3>(s, t, lp, out, scale, m, st); return;
+            case 4: dense_mfma_cfg<B, NL, 2, 4, 8, 1, 2>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 4, 2, 2, 4, 3>(s, t, lp, out, scale, m, st); return;
         }
     }
 }
