@@ -334,7 +334,8 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
                         build_mfma_table(op->mt[w], op->n, s.r_out, s.k_in, op->p, src[w], !s.fwd);
                     }
                     A.ptrs.push_back(op->mt[w].d);
-                    A.bytes += (uint64_t)op->n * op->nl * op->mt[w].rows_pad * op->mt[w].k_pad + kMfmaZeroBytes;
+                    A.bytes += (uint64_t)op->n * op->nl * op->mt[w].rows_pad * op->mt[w].k_pad +
+                               mfma_zero_bytes(op->nl, op->mt[w].rows_pad, op->mt[w].k_pad);
                 }
                 op->d_lp_src = A.alloc<int8_t>(limb_plane_bytes(op->u_src, op->M, b, op->nl));
                 op->d_lp_mid = A.alloc<int8_t>(limb_plane_bytes(op->u_tgt, op->Mp, b, op->nl));

@@ -5,13 +5,20 @@
 
 namespace pgw {
 
-constexpr size_t kMfmaZeroBytes = 1024;  // zero block after the table digits
+constexpr size_t kMfmaZeroBytes = 1024;  // least zero region after the table digits
+
+// Zero bytes after the n tables of an MFMA table: one whole table (so that
+// table index n reads as an all-zero table) and at least 1 KB.
+inline size_t mfma_zero_bytes(uint32_t nl, uint32_t rows_pad, uint32_t k_pad) {
+    const size_t table = (size_t)nl * rows_pad * k_pad;
+    return table > kMfmaZeroBytes ? table : kMfmaZeroBytes;
+}
 
 // Table-stationary digit planes of a synthetic multiplication table for the
 // MFMA sweep: tabl[i][limb][R_pad][K_pad] int8 (balanced base-256 digits).
 struct MfmaTable {
     int8_t* d = nullptr;
-    const int8_t* zero = nullptr;  // 1 KB of zero bytes after the digits (inside the allocation of d)
+    const int8_t* zero = nullptr;  // the zero region after the digits (inside the allocation of d)
     uint32_t rows = 0, k_in = 0, rows_pad = 0, k_pad = 0;
 };
 

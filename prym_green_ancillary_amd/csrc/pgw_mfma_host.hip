@@ -48,7 +48,7 @@ void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, ui
                         (int8_t)(uint8_t)(((u >> (8 * j)) & 0xff) ^ 0x80);
             }
     const size_t digits = host.size();
-    host.resize(digits + kMfmaZeroBytes, 0);
+    host.resize(digits + mfma_zero_bytes(nl, t.rows_pad, t.k_pad), 0);
     PGW_HIP(hipMalloc(&t.d, host.size()));
     PGW_HIP(hipMemcpy(t.d, host.data(), host.size(), hipMemcpyHostToDevice));
     t.zero = t.d + digits;
@@ -126,7 +126,7 @@ void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, ui
 }
 
 // ---------------------------------------------------------------------------
-int mfma_config_count() { return 6; }
+int mfma_config_count() { return 10; }
 
 void launch_dense_mfma_b4(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
                           uint32_t*, const uint32_t*, const ModP&, hipStream_t);

@@ -536,8 +536,23 @@ __device__ __forceinline__ void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
+// Waves per SIMD the direct-to-LDS kernel is compiled for: two workgroups
+// per CU when two copies of its LDS (stages, incidence codes, binomials) fit
+// in the 160 KB of a CU, so one workgroup's staging and fragment reads run
+// under the other's MFMAs; at least two waves per SIMD either way.
+template <int B, int NL, int RT, int CT, int WR, int WC, int STG>
+constexpr int dma_waves_per_simd() {
+    constexpr int rows = WR * RT * 16, tiles_c = WC * CT, slots = tiles_c * (16 / B);
+    constexpr int stage = NL * rows * kLdsStride + tiles_c * NL * 16 * kLdsStride;
+    constexpr int lds = STG * stage + slots * (8 + 4 * kMaxN) + (kMaxN + 1) * kBinW * 8 + 4 * (1 + tiles_c) + 64;
+    constexpr int wgs = 2 * lds <= 160 * 1024 ? 2 : 1;
+    constexpr int w = wgs * WR * WC / 4;
+    return w < 2 ? 2 : w;
+}
+
 template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
-__global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mfma_dma(MfmaSweepArgs a) {
+__global__ __launch_bounds__(64 * WR * WC, (dma_waves_per_simd<B, NL, RT, CT, WR, WC, STG>()))
+void k_dense_mfma_dma(MfmaSweepArgs a) {
     constexpr int NW = WR * WC;
     constexpr int NT = 64 * NW;
     constexpr int NS = 2 * NL - 1;
@@ -609,13 +624,16 @@ __global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mf
         const bool tail = k0 == k_full;
         const uint32_t wshift = tail ? 5u : 6u;  // bytes per chunk row: 32 in the tail, else 64
         static_for<PER>([&](auto q) {
-            const int blk = wave + (int)decltype(q)::value * NW;  // wave-uniform
-            const int row_s = blk * 16 + lane / 4;               // staged row of the stage
-            const int lpc = ((lane % 4) - (row_s >> 2)) & 3;     // logical 16-byte piece
+            constexpr int Q = decltype(q)::value;
+            const int blk = wave + Q * NW;                    // wave-uniform
+            const int row_s = blk * 16 + lane / 4;            // staged row of the stage
+            const int lpc = ((lane % 4) - (row_s >> 2)) & 3;  // logical 16-byte piece
             const int t = (tail && lpc >= 2) ? i2 : i;
             const uint32_t poff = tail ? (lpc & 1) * 16 : lpc * 16;
             const int8_t* src;
-            if (blk < A_BLOCKS) {  // table rows: staged row = j * ROWS + r
+            // blocks Q*NW .. Q*NW+NW-1: all table blocks, all panel blocks, or mixed
+            constexpr bool all_a = (Q + 1) * NW <= A_BLOCKS, all_b = Q * NW >= A_BLOCKS;
+            if (all_a || (!all_b && blk < A_BLOCKS)) {  // table rows: staged row = j * ROWS + r
                 const int r = row_s % ROWS, j = row_s / ROWS;
                 const uint32_t row = wg_row0 + r;
                 const uint32_t off = (t < n && row < a.r_pad)
@@ -754,7 +772,240 @@ __global__ __launch_bounds__(64 * WR * WC, WR * WC >= 8 ? 2 : 1) void k_dense_mf
     mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
 }
 
-// kernel variants of a tile configuration
+// ---------------------------------------------------------------------------
+// register-A variant: table fragments straight from global memory to VGPRs
+//
+// A wave's table rows are its own, so staging them in LDS buys no reuse and
+// costs LDS bandwidth twice (the staging write and the fragment read).  Here
+// every wave loads the RT x NL table fragments of chunk c+1 with plain vector
+// loads (per fragment one coalesced 1 KB block: 16 chunk rows of 64 bytes;
+// in a tail chunk lanes 0-31 read table i's 32-byte rows, lanes 32-63 table
+// i2's) into a second register set while the MFMAs of chunk c run.  Only the
+// digit-plane tile, which all waves of the workgroup read, goes through LDS:
+// direct-to-LDS loads into STG stages, as in the kernel above.  With three
+// stages a wave issues, at chunk c, the fragments of c+1 and then the staging
+// of c+2, so at chunk c+1 it waits for all but its last PER loads.
+// ---------------------------------------------------------------------------
+template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
+__global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepArgs a) {
+    constexpr int NW = WR * WC;
+    constexpr int NT = 64 * NW;
+    constexpr int NS = 2 * NL - 1;
+    constexpr int SPT = 16 / B;
+    constexpr int TILES_C = WC * CT;
+    constexpr int SLOTS = TILES_C * SPT;
+    constexpr int ROWS = WR * RT * 16;
+    constexpr int B_BYTES = TILES_C * NL * 16 * kLdsStride;
+    constexpr int B_BLOCKS = B_BYTES / 1024;
+    constexpr int PER = (B_BLOCKS + NW - 1) / NW;  // staging loads per wave and chunk (at most)
+    static_assert(STG == 2 || STG == 3, "two or three LDS stages");
+    // the partial wait of three stages counts every wave's staging loads
+    static_assert(STG == 2 || B_BLOCKS % NW == 0, "three stages need equal staging per wave");
+
+    __shared__ __align__(1024) int8_t s_st0[B_BYTES];
+    __shared__ __align__(1024) int8_t s_st1[B_BYTES];
+    __shared__ __align__(1024) int8_t s_st2[STG == 3 ? B_BYTES : 16];
+    __shared__ uint64_t s_mask[SLOTS];
+    __shared__ uint32_t s_code[SLOTS * kMaxN];
+    __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
+    __shared__ uint32_t s_masks[1 + TILES_C];
+
+    uint64_t cg;
+    uint32_t rs;
+    if (!mfma_block_tile(a, cg, rs)) return;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int wr = wave / WC, wc = wave % WC;
+    const uint32_t wg_row0 = rs * ROWS;
+    mfma_incidences<FWD, SLOTS, SPT, NT>(a, cg, s_mask, s_code, s_bin, s_masks);
+
+    v4i acc[RT][CT][NS];
+    static_for<RT>([&](auto rt) {
+        static_for<CT>([&](auto ct) { static_for<NS>([&](auto s) { acc[rt][ct][s] = v4i{0, 0, 0, 0}; }); });
+    });
+
+    const uint32_t k_pad = a.k_pad;
+    const uint64_t plane = (uint64_t)NL * B * k_pad;
+    const uint32_t limb_block_b = B * k_pad;
+    const uint32_t limb_block_a = a.r_pad * k_pad;
+    const uint32_t table_stride = NL * limb_block_a;
+    const int n = (int)a.n;
+    const uint32_t k_full = k_pad / 64 * 64;
+    const bool has_tail = k_full < k_pad;
+    const uint32_t any_mask = __builtin_amdgcn_readfirstlane(s_masks[0]);
+    uint32_t on_mask[CT];
+    static_for<CT>([&](auto ct) { on_mask[ct] = __builtin_amdgcn_readfirstlane(s_masks[1 + wc * CT + ct]); });
+    auto next_i = [&](int i) {
+        const uint32_t rest = i < 32 ? any_mask >> i : 0u;
+        return rest ? i + __builtin_ctz(rest) : n;
+    };
+    auto chunk_on = [&](uint32_t mask, int i, int i2, uint32_t k0) {
+        const uint32_t both = (1u << i) | ((k0 == k_full && i2 < n) ? (1u << i2) : 0u);
+        return (mask & both) != 0;
+    };
+    bool row_on[RT];
+    static_for<RT>([&](auto rt) {
+        row_on[rt] = __builtin_amdgcn_readfirstlane((int)(wg_row0 + (wr * RT + rt) * 16 < a.r_pad)) != 0;
+    });
+    const int cc = lane % 16, kq = (lane / 16) * 16;
+
+    const int8_t* zero = a.zero + lane * 16;
+    auto issue_b = [&](int8_t* dst, int i, int i2, uint32_t k0) {
+        const bool tail = k0 == k_full;
+        const uint32_t wshift = tail ? 5u : 6u;
+        static_for<PER>([&](auto q) {
+            const int blk = wave + (int)decltype(q)::value * NW;  // wave-uniform
+            if (B_BLOCKS % NW != 0 && blk >= B_BLOCKS) return;
+            const int rb = blk * 16 + lane / 4;              // staged row: (tile * NL + l) * 16 + c16
+            const int lpc = ((lane % 4) - (rb >> 2)) & 3;    // logical 16-byte piece
+            const int t = (tail && lpc >= 2) ? i2 : i;
+            const uint32_t poff = tail ? (lpc & 1) * 16 : lpc * 16;
+            const int c16 = rb % 16, l = rb / 16 % NL, tile = rb / 16 / NL;
+            const uint32_t cd = t < n ? s_code[(tile * SPT + c16 / B) * kMaxN + t] : 0xffffffffu;
+            const int8_t* src = cd != 0xffffffffu
+                                    ? a.lp + ((uint64_t)(cd >> 31) * a.v_in + (cd & 0x7fffffffu)) * plane +
+                                          ((uint64_t)k0 * B + l * limb_block_b + ((uint32_t)(c16 % B) << wshift) + poff)
+                                    : zero;
+            load_to_lds(src, dst + blk * 1024);
+        });
+    };
+    // this lane's 16 bytes of every table fragment of chunk (i, i2, k0): one
+    // 32-bit lane offset per chunk, the fragments at uniform steps from it
+    // (row tiles 16 chunk rows apart, limbs one limb block apart).  Every load
+    // is in bounds, so none needs a branch: the tail partner i2 may be n, and
+    // table index n is the all-zero table after the digits; a row tile past
+    // R_pad (its MFMAs are skipped) reloads row tile 0, or table rows 0-15 when
+    // the wave has no rows at all.
+    const uint32_t row_a = row_on[0] ? wg_row0 + wr * RT * 16 + cc : (uint32_t)cc;
+    auto load_a = [&](v4i (&fa)[RT][NL], int i, int i2, uint32_t k0) {
+        const bool tail = k0 == k_full;
+        const uint32_t wshift = tail ? 5u : 6u;
+        const int8_t* base = a.tabl + k0 * a.r_pad;
+        const int t = (tail && kq >= 32) ? i2 : i;
+        const uint32_t off = (uint32_t)t * table_stride + (row_a << wshift) + (uint32_t)(tail ? kq & 31 : kq);
+        static_for<RT>([&](auto rt) {
+            const uint32_t roff = row_on[rt] ? (uint32_t)decltype(rt)::value * (16u << wshift) : 0u;
+            static_for<NL>([&](auto j) {
+                fa[rt][j] = *reinterpret_cast<const v4i*>(base + (off + roff + (uint32_t)decltype(j)::value * limb_block_a));
+            });
+        });
+    };
+
+    auto first = [&](int& i, int& i2, uint32_t& k0) {
+        i = next_i(0);
+        if (k_full > 0) {
+            i2 = n;
+            k0 = 0;
+        } else {
+            i2 = i < n ? next_i(i + 1) : n;
+            k0 = k_full;
+        }
+    };
+    auto advance = [&](int& i, int& i2, uint32_t& k0) {
+        if (k0 != k_full) {
+            k0 += 64;
+            if (k0 < k_full) return;
+            i = next_i(i + 1);
+            if (i < n) {
+                k0 = 0;
+                return;
+            }
+            if (!has_tail) return;
+            i = next_i(0);
+            i2 = i < n ? next_i(i + 1) : n;
+            k0 = k_full;
+            return;
+        }
+        i = i2 < n ? next_i(i2 + 1) : n;
+        i2 = i < n ? next_i(i + 1) : n;
+    };
+
+    auto stage = [&](auto st) -> int8_t* {
+        constexpr int X = decltype(st)::value % STG;
+        if constexpr (X == 0) return s_st0;
+        else if constexpr (X == 1) return s_st1;
+        else return s_st2;
+    };
+    v4i fa0[RT][NL], fa1[RT][NL];
+    // chunk c = (i, i2, k0): digit planes in stage ST mod STG, table fragments
+    // in fa<ST mod 2>; c+1 = (j1, j12, q1); false after the last chunk
+    int i, i2, j1, j12;
+    uint32_t k0, q1;
+    auto step = [&](auto ST) {
+        constexpr int S = decltype(ST)::value;
+        v4i(&fcur)[RT][NL] = S % 2 == 0 ? fa0 : fa1;
+        v4i(&fnext)[RT][NL] = S % 2 == 0 ? fa1 : fa0;
+        // this wave's loads of chunk c (staging and fragments) are done (with
+        // three stages the staging of c+1 may stay in flight); after the
+        // barrier every wave's staging of c is, and the stage of c-1 is free
+        if (STG == 3 && j1 < n) wait_vmcnt<PER>(); else wait_vmcnt<0>();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        int j2 = j1, j22 = j12;
+        uint32_t q2 = q1;
+        if (j1 < n) {
+            load_a(fnext, j1, j12, q1);
+            advance(j2, j22, q2);
+            if constexpr (STG == 3) {
+                if (j2 < n) issue_b(stage(std::integral_constant<int, S + 2>{}), j2, j22, q2);
+            } else {
+                issue_b(stage(std::integral_constant<int, S + 1>{}), j1, j12, q1);
+            }
+        }
+        const int8_t* sb = stage(ST);
+        v4i fb[CT][NL];
+        static_for<CT>([&](auto ct) {
+            static_for<NL>([&](auto l) {
+                fb[ct][l] = *reinterpret_cast<const v4i*>(sb + ((wc * CT + ct) * NL + l) * 16 * kLdsStride +
+                                                          lds_off(cc, kq));
+            });
+        });
+        static_for<CT>([&](auto ct) {
+            if (!chunk_on(on_mask[ct], i, i2, k0)) return;
+            static_for<RT>([&](auto rt) {
+                if (!row_on[rt]) return;
+                static_for<NL>([&](auto j) {
+                    static_for<NL>([&](auto l) {
+                        acc[rt][ct][j + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fcur[rt][j], fb[ct][l],
+                                                                                  acc[rt][ct][j + l], 0, 0, 0);
+                    });
+                });
+            });
+        });
+        const bool more = j1 < n;
+        i = j1;
+        i2 = j12;
+        k0 = q1;
+        j1 = j2;
+        j12 = j22;
+        q1 = q2;
+        return more;
+    };
+    first(i, i2, k0);
+    if (i < n) {
+        j1 = i;
+        j12 = i2;
+        q1 = k0;
+        advance(j1, j12, q1);
+        load_a(fa0, i, i2, k0);
+        issue_b(s_st0, i, i2, k0);
+        if constexpr (STG == 3) {
+            if (j1 < n) issue_b(s_st1, j1, j12, q1);
+            while (step(std::integral_constant<int, 0>{}) && step(std::integral_constant<int, 1>{}) &&
+                   step(std::integral_constant<int, 2>{}) && step(std::integral_constant<int, 3>{}) &&
+                   step(std::integral_constant<int, 4>{}) && step(std::integral_constant<int, 5>{})) {
+            }
+        } else {
+            while (step(std::integral_constant<int, 0>{}) && step(std::integral_constant<int, 1>{})) {
+            }
+        }
+    }
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
+}
+
+// kernel variants of a tile configuration: DMA 0 register-staged, 2 or 3
+// direct-to-LDS stages, 4 + s table fragments in registers and s digit-plane
+// stages
 template <int B, int NL, int RT, int CT, int WR, int WC, int DMA = 0>
 static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
                            const uint32_t* scale, const ModP& m, hipStream_t st) {
@@ -794,7 +1045,9 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     require(blocks < (1ULL << 31), "MFMA sweep grid too large");
 #define PGW_MF(K, ...) hipLaunchKernelGGL((K<B, NL, RT, CT, WR, WC, __VA_ARGS__>), dim3((unsigned)blocks), \
                                           dim3(64 * WR * WC), 0, st, a)
-    if constexpr (DMA) {
+    if constexpr (DMA >= 4) {
+        if (s.fwd) PGW_MF(k_dense_mfma_areg, DMA - 4, true); else PGW_MF(k_dense_mfma_areg, DMA - 4, false);
+    } else if constexpr (DMA) {
         if (s.fwd) PGW_MF(k_dense_mfma_dma, DMA, true); else PGW_MF(k_dense_mfma_dma, DMA, false);
     } else {
         if (s.fwd) PGW_MF(k_dense_mfma, true); else PGW_MF(k_dense_mfma, false);
@@ -813,7 +1066,11 @@ static void dense_mfma_pick(int cfg, const DenseSweep& s, const MfmaTable& t, co
             case 2: dense_mfma_cfg<B, NL, 2, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
             case 3: dense_mfma_cfg<B, NL, 2, 2, 4, 2, 3>(s, t, lp, out, scale, m, st); return;
             case 4: dense_mfma_cfg<B, NL, 2, 2, 8, 1, 2>(s, t, lp, out, scale, m, st); return;
-            default: dense_mfma_cfg<B, NL, 2, 2, 2, 4, 3>(s, t, lp, out, scale, m, st); return;
+            case 5: dense_mfma_cfg<B, NL, 2, 2, 2, 4, 3>(s, t, lp, out, scale, m, st); return;
+            case 6: dense_mfma_cfg<B, NL, 2, 2, 2, 2, 2>(s, t, lp, out, scale, m, st); return;
+            case 7: dense_mfma_cfg<B, NL, 2, 2, 8, 1, 6>(s, t, lp, out, scale, m, st); return;
+            case 8: dense_mfma_cfg<B, NL, 2, 2, 4, 1, 6>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 2, 2, 4, 1, 7>(s, t, lp, out, scale, m, st); return;
         }
     } else {
         switch (cfg) {
@@ -822,7 +1079,11 @@ static void dense_mfma_pick(int cfg, const DenseSweep& s, const MfmaTable& t, co
             case 2: dense_mfma_cfg<B, NL, 4, 2, 2, 4>(s, t, lp, out, scale, m, st); return;
             case 3: dense_mfma_cfg<B, NL, 2, 4, 4, 2, 3>(s, t, lp, out, scale, m, st); return;
             case 4: dense_mfma_cfg<B, NL, 2, 4, 8, 1, 2>(s, t, lp, out, scale, m, st); return;
-            default: dense_mfma_cfg<B, NL, 4, 2, 2, 4, 3>(s, t, lp, out, scale, m, st); return;
+            case 5: dense_mfma_cfg<B, NL, 4, 2, 2, 4, 3>(s, t, lp, out, scale, m, st); return;
+            case 6: dense_mfma_cfg<B, NL, 2, 4, 2, 2, 2>(s, t, lp, out, scale, m, st); return;
+            case 7: dense_mfma_cfg<B, NL, 2, 4, 8, 1, 6>(s, t, lp, out, scale, m, st); return;
+            case 8: dense_mfma_cfg<B, NL, 2, 4, 4, 1, 6>(s, t, lp, out, scale, m, st); return;
+            default: dense_mfma_cfg<B, NL, 2, 4, 4, 1, 7>(s, t, lp, out, scale, m, st); return;
         }
     }
 }

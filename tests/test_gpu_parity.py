@@ -17,6 +17,7 @@ import prym_green_ancillary_amd as pg
 pytestmark = pytest.mark.gpu
 
 P31 = 2147483647
+MFMA_CONFIGS = 10  # tile configurations of the MFMA sweep (mfma_config_count in the library)
 
 
 def _log(msg):
@@ -150,7 +151,7 @@ def test_small_synthetic_plain_and_mfma_vs_oracle(torch_cuda, ctx, p, b):
         assert (host(d_out) == o.apply_forward(v, b)).all(), method
 
 
-@pytest.mark.parametrize("cfg", list(range(6)))
+@pytest.mark.parametrize("cfg", list(range(MFMA_CONFIGS)))
 @pytest.mark.parametrize("p,b,n,k,M,Mp", [(P31, 16, 9, 4, 37, 45), (65521, 8, 10, 3, 70, 29),
                                           (P31, 4, 7, 2, 64, 100), (65521, 16, 8, 3, 50, 90)])
 def test_mfma_tile_configs_vs_oracle(torch_cuda, ctx, monkeypatch, cfg, p, b, n, k, M, Mp):
@@ -168,7 +169,7 @@ def test_autotune_keeps_results(torch_cuda, ctx):
     op = pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=pg.METHOD_MFMA, tables=(T, E))
     before = gpu_apply(torch_cuda, op, v)
     choice, ms = op.autotune(2)
-    assert len(choice) == 4 and all(0 <= c < 6 for c in choice) and all(t > 0 for t in ms)
+    assert len(choice) == 4 and all(0 <= c < MFMA_CONFIGS for c in choice) and all(t > 0 for t in ms)
     assert (gpu_apply(torch_cuda, op, v) == before).all()
 
 
