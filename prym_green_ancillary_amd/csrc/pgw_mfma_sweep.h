@@ -809,6 +809,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
     __shared__ uint32_t s_code[SLOTS * kMaxN];
     __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
     __shared__ uint32_t s_masks[1 + TILES_C];
+    __shared__ const int8_t* s_bptr[SLOTS * (kMaxN + 1)];  // digit plane of panel g's table-t neighbour
 
     uint64_t cg;
     uint32_t rs;
@@ -817,6 +818,21 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
     mfma_incidences<FWD, SLOTS, SPT, NT>(a, cg, s_mask, s_code, s_bin, s_masks);
+    // s_bptr[g * (kMaxN + 1) + t]: base of the digit plane (sign and limb 0)
+    // that panel g reads at table t, or the zero table when g is not incident
+    // to t (t = n included: the missing partner of a tail pair).  The zero
+    // table spans NL * R_pad * K_pad >= NL * B * K_pad bytes, one whole plane.
+    {
+        const uint64_t plane_b = (uint64_t)NL * B * a.k_pad;
+        const int np1 = (int)a.n + 1;
+        for (int idx = tid; idx < SLOTS * np1; idx += NT) {
+            const int g = idx / np1, t = idx % np1;
+            const uint32_t cd = t < (int)a.n ? s_code[g * kMaxN + t] : 0xffffffffu;
+            s_bptr[g * (kMaxN + 1) + t] =
+                cd != 0xffffffffu ? a.lp + ((uint64_t)(cd >> 31) * a.v_in + (cd & 0x7fffffffu)) * plane_b : a.zero;
+        }
+        __syncthreads();
+    }
 
     v4i acc[RT][CT][NS];
     static_for<RT>([&](auto rt) {
@@ -848,23 +864,39 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
     });
     const int cc = lane % 16, kq = (lane / 16) * 16;
 
-    const int8_t* zero = a.zero + lane * 16;
+    // Staging load q of this lane: block blk = wave + q * NW of the stage
+    // holds staged rows (tile * NL + l) * 16 + c16; the lane fills logical
+    // 16-byte piece lpc of row c16, i.e. panel g = tile * SPT + c16 / B, limb
+    // l, column c16 % B, chunk bytes 16 lpc (a tail chunk: table i's 32 bytes
+    // in pieces 0-1, table i2's in pieces 2-3).  Offsets within the plane:
+    int b_g[PER];
+    bool b_hi[PER];                         // tail: piece of table i2
+    uint32_t b_full[PER], b_tail[PER];      // plane offset without k0 * B, full / tail chunk
+    static_for<PER>([&](auto q) {
+        const int blk = wave + (int)decltype(q)::value * NW;
+        const int rb = blk * 16 + lane / 4;
+        const int lpc = ((lane % 4) - (rb >> 2)) & 3;
+        const int c16 = rb % 16, l = rb / 16 % NL, tile = rb / 16 / NL;
+        b_g[q] = (tile * SPT + c16 / B) * (kMaxN + 1);
+        b_hi[q] = lpc >= 2;
+        b_full[q] = l * limb_block_b + ((uint32_t)(c16 % B) << 6) + lpc * 16;
+        b_tail[q] = l * limb_block_b + ((uint32_t)(c16 % B) << 5) + (lpc & 1) * 16;
+    });
+    // plane bases of the table whose full chunks are being staged (read from
+    // LDS once per table, not once per chunk)
+    const int8_t* b_base[PER];
+    int b_table = -1;
     auto issue_b = [&](int8_t* dst, int i, int i2, uint32_t k0) {
         const bool tail = k0 == k_full;
-        const uint32_t wshift = tail ? 5u : 6u;
+        if (!tail && i != b_table) {
+            static_for<PER>([&](auto q) { b_base[q] = s_bptr[b_g[q] + i]; });
+            b_table = i;
+        }
         static_for<PER>([&](auto q) {
             const int blk = wave + (int)decltype(q)::value * NW;  // wave-uniform
             if (B_BLOCKS % NW != 0 && blk >= B_BLOCKS) return;
-            const int rb = blk * 16 + lane / 4;              // staged row: (tile * NL + l) * 16 + c16
-            const int lpc = ((lane % 4) - (rb >> 2)) & 3;    // logical 16-byte piece
-            const int t = (tail && lpc >= 2) ? i2 : i;
-            const uint32_t poff = tail ? (lpc & 1) * 16 : lpc * 16;
-            const int c16 = rb % 16, l = rb / 16 % NL, tile = rb / 16 / NL;
-            const uint32_t cd = t < n ? s_code[(tile * SPT + c16 / B) * kMaxN + t] : 0xffffffffu;
-            const int8_t* src = cd != 0xffffffffu
-                                    ? a.lp + ((uint64_t)(cd >> 31) * a.v_in + (cd & 0x7fffffffu)) * plane +
-                                          ((uint64_t)k0 * B + l * limb_block_b + ((uint32_t)(c16 % B) << wshift) + poff)
-                                    : zero;
+            const int8_t* src = tail ? s_bptr[b_g[q] + (b_hi[q] ? i2 : i)] + (k0 * B + b_tail[q])
+                                     : b_base[q] + (k0 * B + b_full[q]);
             load_to_lds(src, dst + blk * 1024);
         });
     };
