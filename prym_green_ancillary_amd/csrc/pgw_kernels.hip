@@ -446,10 +446,25 @@ void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uin
     PGW_CHECK_LAUNCH("scale rows");
 }
 
+// v mod p for v < 2^63.  M31 (p = 2^31 - 1): 2^31 == 1, so two folds of the
+// high bits onto the low 31 bits leave v' < 2^31 + 2 and one conditional
+// subtraction finishes; otherwise 64-bit Barrett.
+template <bool M31>
+__device__ __forceinline__ uint32_t red63(uint64_t v, const ModP& m) {
+    if constexpr (M31) {
+        uint64_t r = (v & 0x7fffffffULL) + (v >> 31);  // < 2^32 + 2^31
+        r = (r & 0x7fffffffULL) + (r >> 31);           // < 2^31 + 3
+        return (uint32_t)(r >= 0x7fffffffULL ? r - 0x7fffffffULL : r);
+    } else {
+        return mod_u64(v, m);
+    }
+}
+
 // one round of paired 2x2 maps; forward [[1,a],[b,1+ab]], transpose [[1,b],[a,1+ab]]
 // (cuda/cyclic_cuda_helpers.h:109-143).  `mid` fuses forward, Dr and transpose
 // of the same round: the 2x2 block P_r^T diag(dr_f, dr_s) P_r on each pair.
-template <int B, int MODE>  // 0 forward, 1 transpose, 2 fused middle
+// Every sum below is u*x + v*y with residues below 2^31, so below 2^63.
+template <int B, int MODE, bool M31>  // MODE 0 forward, 1 transpose, 2 fused middle
 __global__ __launch_bounds__(256) void k_rowmix(const pgw_rowmix_pair* __restrict__ pairs,
                                                 uint64_t count, uint32_t* __restrict__ block,
                                                 const uint32_t* __restrict__ dr, ModP m) {
@@ -459,21 +474,22 @@ __global__ __launch_bounds__(256) void k_rowmix(const pgw_rowmix_pair* __restric
     if (threadIdx.x / B >= RPB || q >= count) return;
     const pgw_rowmix_pair pr = pairs[q];
     const uint32_t p = m.p;
-    const uint32_t opab = addmod(1u, mulmod(pr.a, pr.b, m), p);
+    const uint32_t opab = addmod(1u, red63<M31>((uint64_t)pr.a * pr.b, m), p);
     uint32_t* fx = block + (uint64_t)pr.first * B + c;
     uint32_t* sy = block + (uint64_t)pr.second * B + c;
     uint64_t x = *fx, y = *sy;
     uint32_t nx, ny;
     if (MODE == 1) {
-        nx = mod_u64(x + (uint64_t)pr.b * y, m);
-        ny = mod_u64((uint64_t)pr.a * x + (uint64_t)opab * y, m);
+        nx = red63<M31>(x + (uint64_t)pr.b * y, m);
+        ny = red63<M31>((uint64_t)pr.a * x + (uint64_t)opab * y, m);
     } else {
-        nx = mod_u64(x + (uint64_t)pr.a * y, m);
-        ny = mod_u64((uint64_t)pr.b * x + (uint64_t)opab * y, m);
+        nx = red63<M31>(x + (uint64_t)pr.a * y, m);
+        ny = red63<M31>((uint64_t)pr.b * x + (uint64_t)opab * y, m);
         if (MODE == 2) {
-            const uint64_t sx = mulmod(nx, dr[pr.first], m), sy2 = mulmod(ny, dr[pr.second], m);
-            nx = mod_u64(sx + (uint64_t)pr.b * sy2, m);
-            ny = mod_u64((uint64_t)pr.a * sx + (uint64_t)opab * sy2, m);
+            const uint64_t sx = red63<M31>((uint64_t)nx * dr[pr.first], m);
+            const uint64_t sy2 = red63<M31>((uint64_t)ny * dr[pr.second], m);
+            nx = red63<M31>(sx + (uint64_t)pr.b * sy2, m);
+            ny = red63<M31>((uint64_t)pr.a * sx + (uint64_t)opab * sy2, m);
         }
     }
     *fx = nx;
@@ -487,23 +503,31 @@ __global__ void k_scale_one_row(uint32_t* __restrict__ block, uint64_t row,
     if (threadIdx.x < b) block[row * b + threadIdx.x] = mulmod(block[row * b + threadIdx.x], dr[row], m);
 }
 
-template <int B>
-static void target_precond_b(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pc,
-                             uint32_t rounds, const uint32_t* dr, int64_t unpaired_row,
-                             const ModP& m, hipStream_t st) {
+template <int B, bool M31>
+static void target_precond_bm(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pc,
+                              uint32_t rounds, const uint32_t* dr, int64_t unpaired_row,
+                              const ModP& m, hipStream_t st) {
     dim3 grid(grid_for(pc, 256 / B)), blk(256);
     for (uint32_t r = 0; r + 1 < rounds; ++r)
-        hipLaunchKernelGGL((k_rowmix<B, 0>), grid, blk, 0, st, pairs + (uint64_t)r * pc, pc,
+        hipLaunchKernelGGL((k_rowmix<B, 0, M31>), grid, blk, 0, st, pairs + (uint64_t)r * pc, pc,
                            block, dr, m);
-    hipLaunchKernelGGL((k_rowmix<B, 2>), grid, blk, 0, st, pairs + (uint64_t)(rounds - 1) * pc,
+    hipLaunchKernelGGL((k_rowmix<B, 2, M31>), grid, blk, 0, st, pairs + (uint64_t)(rounds - 1) * pc,
                        pc, block, dr, m);
     if (unpaired_row >= 0)
         hipLaunchKernelGGL(k_scale_one_row, dim3(1), dim3(64), 0, st, block, (uint64_t)unpaired_row,
                            dr, (uint32_t)B, m);
     for (uint32_t r = rounds - 1; r-- > 0;)
-        hipLaunchKernelGGL((k_rowmix<B, 1>), grid, blk, 0, st, pairs + (uint64_t)r * pc, pc,
+        hipLaunchKernelGGL((k_rowmix<B, 1, M31>), grid, blk, 0, st, pairs + (uint64_t)r * pc, pc,
                            block, dr, m);
     PGW_CHECK_LAUNCH("row mixing");
+}
+
+template <int B>
+static void target_precond_b(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pc,
+                             uint32_t rounds, const uint32_t* dr, int64_t unpaired_row,
+                             const ModP& m, hipStream_t st) {
+    if (m.p == 0x7fffffffu) target_precond_bm<B, true>(block, pairs, pc, rounds, dr, unpaired_row, m, st);
+    else target_precond_bm<B, false>(block, pairs, pc, rounds, dr, unpaired_row, m, st);
 }
 
 void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pair_count,
