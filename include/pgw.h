@@ -43,7 +43,13 @@ enum pgw_kind {
     PGW_KIND_SYNTHETIC = 1,
     /* the reference's eliminated cyclic-sector operator
      * F = [ -D_(m-1) R D_m | D_(m-1) B ]  (cuda/cuprym_cyclic.cu:2330-2359) */
-    PGW_KIND_CYCLIC_ELIMINATED = 2
+    PGW_KIND_CYCLIC_ELIMINATED = 2,
+    /* deformation variants of that operator (run_deformation_eliminated_variant,
+     * cuda/cuprym_deformation.cu:2554-2855): augmented G = [F | -dense],
+     * replacement G = [F without base columns drop_cols[0] < drop_cols[1] | dense];
+     * source columns: base (or kept base) columns, then the dense columns. */
+    PGW_KIND_CYCLIC_AUGMENTED = 3,
+    PGW_KIND_CYCLIC_REPLACEMENT = 4
 };
 
 enum pgw_method {
@@ -94,6 +100,13 @@ typedef struct {
     const uint16_t* kernel_weights;
     const uint32_t* right_inverse;   /* [genus][n]          */
     const uint32_t* kernel_basis;    /* [genus][kernel_dim] */
+    /* PGW_KIND_CYCLIC_AUGMENTED / _REPLACEMENT (the --dense-file and
+     * --drop-columns inputs, cuda/cuprym_deformation.cu:2600-2619): the dense
+     * block, target rows x dense_cols row-major, dense_cols = 2 as the reference
+     * requires; the two base columns the replacement drops, ascending. */
+    uint32_t dense_cols;
+    const uint32_t* dense_block;
+    uint32_t drop_cols[2];
 } pgw_operator_desc;
 
 typedef struct {

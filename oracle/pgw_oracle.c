@@ -563,6 +563,14 @@ struct orc_op {
     size_t z_count;
     uint32_t* z_off; uint32_t* z_rank; uint16_t* z_kappa;
     size_t x_cols;
+    /* deformation variant of the cyclic operator: 0 none, 1 augmented
+     * G = [F0 | -dense], 2 replacement G = [F0 without columns drop0 < drop1 |
+     * dense]; dense is target rows x dense_cols, row-major */
+    int deform;
+    const uint32_t* dense;
+    size_t dense_cols;
+    size_t drop0, drop1;
+    size_t base_cols;
 };
 
 orc_op* orc_synthetic_create(uint32_t n, uint32_t k, uint32_t M, uint32_t Mp, uint32_t p,
@@ -642,6 +650,27 @@ orc_op* orc_cyclic_create(uint32_t genus, uint32_t r, uint32_t p, uint32_t secto
     op->cols = op->dm.cols + op->z_count;
     op->rows = op->d2.rows;
     return op;
+}
+
+/* The augmented and replacement operators of run_deformation_eliminated_variant
+ * (cuda/cuprym_deformation.cu:2554-2855) on an eliminated cyclic operator:
+ * augmented G = [F0 | -dense] (forward: dense axpy with SUBTRACT, :2787-2795),
+ * replacement G = [F0 restricted to the base columns other than drop0 < drop1 |
+ * dense] (insert/gather complement, :2796-2855).  Source columns: base then
+ * dense for augmented, kept base columns then dense for replacement. */
+int orc_cyclic_deformation(orc_op* op, int variant, const uint32_t* dense, size_t dense_cols,
+                           size_t drop0, size_t drop1) {
+    if (op->kind != ORC_CYCLIC || op->deform || (variant != 1 && variant != 2) || dense_cols == 0)
+        return -1;
+    op->base_cols = op->cols;
+    if (variant == 2 && !(drop0 < drop1 && drop1 < op->base_cols)) return -1;
+    op->deform = variant;
+    op->dense = dense;
+    op->dense_cols = dense_cols;
+    op->drop0 = drop0;
+    op->drop1 = drop1;
+    op->cols = (variant == 1 ? op->base_cols : op->base_cols - 2) + dense_cols;
+    return 0;
 }
 
 void orc_set_preconditioners(orc_op* op, const uint32_t* dr, const uint32_t* dc,
@@ -724,7 +753,7 @@ static void elim_y(const orc_op* op, const uint32_t* z, const uint32_t* t, uint3
 }
 
 /* forward F*(in): input already Dc-scaled */
-static void forward_core(orc_op* op, const uint32_t* scaled, uint32_t* out, size_t b) {
+static void forward_base(orc_op* op, const uint32_t* scaled, uint32_t* out, size_t b) {
     uint32_t p = op->p;
     if (op->kind == ORC_SYNTHETIC) {
         phi_op* d = &op->d_op;
@@ -746,7 +775,7 @@ static void forward_core(orc_op* op, const uint32_t* scaled, uint32_t* out, size
 }
 
 /* transpose F^T*(in), no Dc */
-static void backward_core(orc_op* op, const uint32_t* in, uint32_t* out, size_t b) {
+static void backward_base(orc_op* op, const uint32_t* in, uint32_t* out, size_t b) {
     uint32_t p = op->p;
     if (op->kind == ORC_SYNTHETIC) {
         phi_op* d = &op->d_op;
@@ -789,6 +818,71 @@ static void backward_core(orc_op* op, const uint32_t* in, uint32_t* out, size_t 
         free(h);
         free(tadj);
     }
+}
+
+/* base column of kept column j (replacement: the base columns in order with
+ * drop0 and drop1 left out) */
+static size_t kept_to_base(const orc_op* op, size_t j) {
+    size_t f = j;
+    if (f >= op->drop0) ++f;
+    if (f >= op->drop1) ++f;
+    return f;
+}
+
+/* G*(in) for the deformation variants (input already scaled) */
+static void forward_core(orc_op* op, const uint32_t* scaled, uint32_t* out, size_t b) {
+    if (!op->deform) {
+        forward_base(op, scaled, out, b);
+        return;
+    }
+    const uint32_t p = op->p;
+    const size_t base = op->base_cols, dc = op->dense_cols;
+    const size_t lead = op->deform == 1 ? base : base - 2;  /* columns before the dense block */
+    const uint32_t* extra = scaled + lead * b;
+    if (op->deform == 1) {
+        forward_base(op, scaled, out, b);
+    } else {
+        uint32_t* full = (uint32_t*)calloc(base * b + 1, sizeof(uint32_t));
+        for (size_t j = 0; j < lead; ++j)
+            memcpy(full + kept_to_base(op, j) * b, scaled + j * b, b * sizeof(uint32_t));
+        forward_base(op, full, out, b);
+        free(full);
+    }
+    for (size_t r = 0; r < op->rows; ++r)
+        for (size_t c = 0; c < b; ++c) {
+            u128 sum = 0;
+            for (size_t j = 0; j < dc; ++j) sum += (uint64_t)op->dense[r * dc + j] * extra[j * b + c];
+            uint64_t t = (uint64_t)(sum % p);
+            uint64_t v = op->deform == 1 ? out[r * b + c] + (p - t) : out[r * b + c] + t;
+            out[r * b + c] = (uint32_t)(v % p);
+        }
+}
+
+/* G^T*(in) for the deformation variants, no Dc */
+static void backward_core(orc_op* op, const uint32_t* in, uint32_t* out, size_t b) {
+    if (!op->deform) {
+        backward_base(op, in, out, b);
+        return;
+    }
+    const uint32_t p = op->p;
+    const size_t base = op->base_cols, dc = op->dense_cols;
+    const size_t lead = op->deform == 1 ? base : base - 2;
+    if (op->deform == 1) {
+        backward_base(op, in, out, b);
+    } else {
+        uint32_t* full = (uint32_t*)malloc((base * b + 1) * sizeof(uint32_t));
+        backward_base(op, in, full, b);
+        for (size_t j = 0; j < lead; ++j)
+            memcpy(out + j * b, full + kept_to_base(op, j) * b, b * sizeof(uint32_t));
+        free(full);
+    }
+    for (size_t j = 0; j < dc; ++j)
+        for (size_t c = 0; c < b; ++c) {
+            u128 sum = 0;
+            for (size_t r = 0; r < op->rows; ++r) sum += (uint64_t)op->dense[r * dc + j] * in[r * b + c];
+            uint32_t t = (uint32_t)(sum % p);
+            out[(lead + j) * b + c] = op->deform == 1 ? (t ? p - t : 0) : t;
+        }
 }
 
 void orc_apply_forward(orc_op* op, const uint32_t* in, uint32_t* out, size_t b) {

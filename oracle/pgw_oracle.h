@@ -87,6 +87,14 @@ orc_op* orc_cyclic_create(uint32_t genus, uint32_t r, uint32_t p, uint32_t secto
                           const uint16_t* kernel_weights, const uint32_t* right_inverse,
                           const uint32_t* kernel_basis);
 /* Set preconditioners / rowmix after the dims are known (cyclic op). */
+/* Deformation variants of an eliminated cyclic operator
+ * (cuda/cuprym_deformation.cu:2554-2855): variant 1 augmented [F0 | -dense],
+ * variant 2 replacement [F0 without base columns drop0 < drop1 | dense];
+ * dense is rows x dense_cols (kept by reference).  Call before
+ * orc_set_preconditioners; the operator's column count becomes that of G.
+ * Returns 0, or -1 for an invalid request. */
+int orc_cyclic_deformation(orc_op* op, int variant, const uint32_t* dense, size_t dense_cols,
+                           size_t drop0, size_t drop1);
 void orc_set_preconditioners(orc_op* op, const uint32_t* dr, const uint32_t* dc,
                              size_t rowmix_rounds, const uint32_t* rowmix_pairs);
 void orc_op_dims(const orc_op* op, size_t* rows, size_t* cols);

@@ -37,6 +37,8 @@ DEFAULT_MAX_NLEN_BUFFER = 40
 
 KIND_SYNTHETIC = 1
 KIND_CYCLIC = 2
+KIND_CYCLIC_AUGMENTED = 3
+KIND_CYCLIC_REPLACEMENT = 4
 METHOD_AUTO, METHOD_PLAIN, METHOD_MFMA = 0, 1, 2
 
 
@@ -68,6 +70,8 @@ class OperatorDesc(ctypes.Structure):
         ("w_index", ctypes.c_uint32), ("w_weight", ctypes.c_uint32), ("kernel_dim", ctypes.c_uint32),
         ("kernel_weights", ctypes.c_void_p), ("right_inverse", ctypes.c_void_p),
         ("kernel_basis", ctypes.c_void_p),
+        ("dense_cols", ctypes.c_uint32), ("dense_block", ctypes.c_void_p),
+        ("drop_cols", ctypes.c_uint32 * 2),
     ]
 
 
@@ -376,6 +380,26 @@ class Operator:
     @classmethod
     def cyclic(cls, ctx, inst: dict, sector: int, block: int, dr, dc, rounds=0, pairs=None):
         d, keep = cls.cyclic_desc(inst, sector, block)
+        return cls(ctx, d, dr, dc, rounds, pairs, keep)
+
+    @staticmethod
+    def deformation_desc(inst: dict, sector: int, block: int, variant: str, dense, drops=(0, 0)):
+        """Augmented [F | -dense] or replacement [F without drops | dense]
+        (cuda/cuprym_deformation.cu:2554-2855); dense is rows x 2."""
+        d, arrays = Operator.cyclic_desc(inst, sector, block)
+        d.kind = {"augmented": KIND_CYCLIC_AUGMENTED, "replacement": KIND_CYCLIC_REPLACEMENT}[variant]
+        dense = np.ascontiguousarray(np.asarray(dense, dtype=np.uint32))
+        if dense.ndim != 2:
+            raise ValueError("dense block must be 2-dimensional (target rows x columns)")
+        arrays["dense"] = dense
+        d.dense_cols, d.dense_block = dense.shape[1], dense.ctypes.data
+        d.drop_cols[0], d.drop_cols[1] = int(drops[0]), int(drops[1])
+        return d, arrays
+
+    @classmethod
+    def deformation(cls, ctx, inst: dict, sector: int, block: int, variant: str, dense, drops,
+                    dr, dc, rounds=0, pairs=None):
+        d, keep = cls.deformation_desc(inst, sector, block, variant, dense, drops)
         return cls(ctx, d, dr, dc, rounds, pairs, keep)
 
     @staticmethod

@@ -120,11 +120,23 @@ void forward_into_target(pgw_op* op, const uint32_t* x, hipStream_t st) {
             run_sweep(op, 1, op->d_mid, nullptr, op->d_target, nullptr, st);
         }
     } else {
-        launch_scale_rows(x, op->d_dc, op->d_scaled, op->cols, op->b, m, st);
+        // Dc-scaled input: the base rows then, for a deformation variant, the
+        // dense-column rows (replacement: kept columns carried to their base
+        // positions, cuda/cuprym_deformation.cu:2787-2821)
+        if (op->kind == PGW_KIND_CYCLIC_REPLACEMENT)
+            launch_complement_scale(x, op->d_dc, op->d_scaled, op->base_cols, op->drop0, op->drop1,
+                                    op->dense_cols, op->b, m, st);
+        else
+            launch_scale_rows(x, op->d_dc, op->d_scaled, op->cols, op->b, m, st);
         launch_cyclic_fwd(op->dm, op->d_scaled, op->d_t, nullptr, op->b, m, st);
         launch_elim_y(op->dm, op->d2, op->d_zoff, op->d_zkappa, op->d_R, op->d_Bk, op->n_v, op->kdim,
                       op->d_scaled + op->x_cols * op->b, op->d_t, op->d_y, op->b, m, st);
         launch_cyclic_fwd(op->d2, op->d_y, op->d_target, nullptr, op->b, m, st);
+        // [F | -dense] (augmented) or [F' | dense] (replacement)
+        if (op->dense_cols)
+            launch_dense_cols_axpy(op->d_target, op->d_dense, op->d_scaled + op->base_cols * op->b,
+                                   op->rows, op->dense_cols, op->b,
+                                   op->kind == PGW_KIND_CYCLIC_AUGMENTED, m, st);
     }
 }
 
@@ -142,12 +154,25 @@ void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
             run_sweep(op, 3, op->d_mid, nullptr, out, op->d_dc, st);
         }
     } else {
+        // the base adjoint lands in `out` (plain and augmented) or, for the
+        // replacement, in a base-sized block whose kept rows are gathered
+        uint32_t* base_out = op->kind == PGW_KIND_CYCLIC_REPLACEMENT ? op->d_base_adj : out;
         launch_cyclic_trn(op->d2, op->d_target, op->d_h, nullptr, op->b, m, st);
         launch_elim_rt_h(op->dm, op->d2, op->d_R, op->n_v, op->d_h, op->d_tadj, op->b, m, st);
         // -x_adj scaled by Dc: fold the sign into the scale (p - Dc)
-        launch_cyclic_trn(op->dm, op->d_tadj, out, op->d_negdc, op->b, m, st);
+        launch_cyclic_trn(op->dm, op->d_tadj, base_out, op->d_base_negdc, op->b, m, st);
         launch_elim_bt_h(op->d2, op->z_count, op->d_zrank, op->d_zkappa, op->d_Bk, op->kdim, op->d_h,
-                         out + op->x_cols * op->b, op->d_dc + op->x_cols, op->b, m, st);
+                         base_out + op->x_cols * op->b, op->d_base_dc + op->x_cols, op->b, m, st);
+        if (op->dense_cols) {
+            const bool aug = op->kind == PGW_KIND_CYCLIC_AUGMENTED;
+            const uint64_t lead = aug ? op->base_cols : op->base_cols - 2;  // rows before the dense ones
+            if (!aug)
+                launch_complement_gather(op->d_base_adj, out, op->base_cols, op->drop0, op->drop1, op->b, st);
+            // dense^T (target), negated for [F | -dense], scaled by Dc
+            // (cuda/cuprym_deformation.cu:2822-2855)
+            launch_dense_cols_transpose(op->d_dense, op->d_target, op->rows, op->dense_cols, op->b,
+                                        op->d_dc, lead, out, aug, op->d_dense_scratch, m, st);
+        }
     }
 }
 
@@ -184,6 +209,27 @@ void synthetic_dims(const pgw_operator_desc* d, uint64_t& rows, uint64_t& cols, 
     cols = us * d->m_src;
     rows = cols;
     mid = ut * d->m_tgt;
+}
+
+bool is_deformation(uint32_t kind) {
+    return kind == PGW_KIND_CYCLIC_AUGMENTED || kind == PGW_KIND_CYCLIC_REPLACEMENT;
+}
+
+// Source columns of a deformation variant over an eliminated base with
+// `base` columns, after the checks of run_deformation_eliminated_variant
+// (cuda/cuprym_deformation.cu:2600-2623, load_drop_columns_file :452-472):
+// two dense columns; for the replacement two distinct base columns, which
+// are returned ascending.
+uint64_t deformation_cols(const pgw_operator_desc* d, uint64_t base, uint32_t& drop0, uint32_t& drop1) {
+    require(d->dense_cols == 2, "deformation operators take a dense block of two columns");
+    require(d->dense_block != nullptr, "deformation operator needs its dense block");
+    drop0 = drop1 = 0;
+    if (d->kind == PGW_KIND_CYCLIC_AUGMENTED) return base + d->dense_cols;
+    drop0 = std::min(d->drop_cols[0], d->drop_cols[1]);
+    drop1 = std::max(d->drop_cols[0], d->drop_cols[1]);
+    require(drop1 < base, "drop-column index is outside the base column range");
+    require(drop0 != drop1, "drop-column indices must be distinct");
+    return base - 2 + d->dense_cols;
 }
 
 }  // namespace
@@ -266,11 +312,13 @@ int pgw_operator_dims(const pgw_operator_desc* desc, uint64_t* rows, uint64_t* c
         uint64_t r = 0, c = 0, mid = 0;
         if (desc->kind == PGW_KIND_SYNTHETIC) {
             synthetic_dims(desc, r, c, mid);
-        } else if (desc->kind == PGW_KIND_CYCLIC_ELIMINATED) {
+        } else if (desc->kind == PGW_KIND_CYCLIC_ELIMINATED || is_deformation(desc->kind)) {
             CyclicLayout L;
             build_cyclic_layout(L, *desc);
             r = L.rows;
             c = L.cols;
+            uint32_t d0, d1;
+            if (is_deformation(desc->kind)) c = deformation_cols(desc, c, d0, d1);
         } else {
             throw Error(PGW_ERR_INVALID, "unknown operator kind");
         }
@@ -360,7 +408,7 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
                 op->d_scaled = A.alloc<uint32_t>(op->cols * b);
             }
             op->d_mid = A.alloc<uint32_t>(op->mid * b);
-        } else if (desc->kind == PGW_KIND_CYCLIC_ELIMINATED) {
+        } else if (desc->kind == PGW_KIND_CYCLIC_ELIMINATED || is_deformation(desc->kind)) {
             require(desc->method != PGW_METHOD_MFMA,
                     "the cyclic operator's character blocks run on the plain path", PGW_ERR_UNSUPPORTED);
             op->method = PGW_METHOD_PLAIN;
@@ -368,6 +416,19 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             build_cyclic_layout(L, *desc);
             op->rows = L.rows;
             op->cols = L.cols;
+            op->base_cols = L.cols;
+            if (is_deformation(desc->kind)) {
+                op->cols = deformation_cols(desc, L.cols, op->drop0, op->drop1);
+                op->dense_cols = desc->dense_cols;
+                const uint64_t cnt = op->rows * op->dense_cols;
+                uint32_t dlo = 0, dhi = 0;
+                min_max_u32(desc->dense_block, cnt, dlo, dhi);
+                require(cnt == 0 || dhi < op->p, "dense block entries must be reduced modulo p");
+                op->d_dense = A.upload(desc->dense_block, cnt);
+                op->d_dense_scratch = A.alloc<uint64_t>(dense_cols_scratch_bytes(op->dense_cols, b) / 8);
+                if (desc->kind == PGW_KIND_CYCLIC_REPLACEMENT)
+                    op->d_base_adj = A.alloc<uint32_t>(op->base_cols * b);
+            }
             op->mid = L.d2.col_rank.size();
             op->dm = upload_phi(A, L.dm);
             op->d2 = upload_phi(A, L.d2);
@@ -383,7 +444,8 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             op->d_Bk = A.upload(desc->kernel_basis, (uint64_t)g * desc->kernel_dim);
             op->x_cols = L.x_cols;
             op->z_count = L.z_count;
-            op->d_scaled = A.alloc<uint32_t>(op->cols * b);
+            // scaled input: the base rows, then the dense-column rows of a variant
+            op->d_scaled = A.alloc<uint32_t>((op->base_cols + op->dense_cols) * b);
             op->d_t = A.alloc<uint32_t>(L.dm.row_rank.size() * b);
             op->d_y = A.alloc<uint32_t>(op->mid * b);
             op->d_h = A.alloc<uint32_t>(op->mid * b);
@@ -402,6 +464,21 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
         std::vector<uint32_t> negdc(op->cols);
         for (uint64_t q = 0; q < op->cols; ++q) negdc[q] = op->p - h_col_precond[q];
         op->d_negdc = A.upload(negdc);
+        // the base operator's source preconditioner: Dc itself, or for the
+        // replacement Dc carried to the base positions of the kept columns
+        // (0 at the two dropped columns, whose adjoint rows are discarded)
+        op->d_base_dc = op->d_dc;
+        op->d_base_negdc = op->d_negdc;
+        if (op->kind == PGW_KIND_CYCLIC_REPLACEMENT) {
+            std::vector<uint32_t> bdc(op->base_cols, 0), bneg(op->base_cols, 0);
+            for (uint64_t f = 0, j = 0; f < op->base_cols; ++f) {
+                if (f == op->drop0 || f == op->drop1) continue;
+                bdc[f] = h_col_precond[j++];
+                bneg[f] = op->p - bdc[f];
+            }
+            op->d_base_dc = A.upload(bdc);
+            op->d_base_negdc = A.upload(bneg);
+        }
         op->rounds = rowmix_rounds;
         op->pair_count = op->rows / 2;
         if (rowmix_rounds > 0 && op->pair_count > 0) {

@@ -385,7 +385,7 @@ using Clock = std::chrono::steady_clock;
 // Owned copy of an operator description and the arrays it points to.
 struct OwnedDesc {
     pgw_operator_desc d{};
-    std::vector<uint32_t> table_d, table_e, mu, right_inverse, kernel_basis;
+    std::vector<uint32_t> table_d, table_e, mu, right_inverse, kernel_basis, dense;
     std::vector<uint16_t> wv, wa0, wa1, kw;
 
     void copy_from(const pgw_operator_desc& s) {
@@ -415,6 +415,12 @@ struct OwnedDesc {
             d.kernel_weights = kw.data();
             d.right_inverse = right_inverse.data();
             d.kernel_basis = kernel_basis.data();
+            if (s.kind == PGW_KIND_CYCLIC_AUGMENTED || s.kind == PGW_KIND_CYCLIC_REPLACEMENT) {
+                uint64_t rows = 0;
+                if (pgw_operator_dims(&s, &rows, nullptr) != PGW_OK) throw Error(PGW_ERR_INVALID, pgw_last_error());
+                grab(dense, s.dense_block, rows * s.dense_cols);
+                d.dense_block = dense.data();
+            }
         }
     }
 };
@@ -467,6 +473,9 @@ uint64_t estimate_job_bytes(const pgw_operator_desc& d, uint64_t rows, uint64_t 
         bytes += 2 * ut2 * 4 * b * (((uint64_t)d.m_tgt + 31) / 32 * 32);  // mid digit planes
     } else {
         bytes += 6 * (rows + cols) * b * 4 + 64ull * (rows + cols);
+        if (d.kind == PGW_KIND_CYCLIC_AUGMENTED || d.kind == PGW_KIND_CYCLIC_REPLACEMENT)
+            bytes += rows * d.dense_cols * 4 + 2 * (cols + d.dense_cols) * b * 4 +  // dense, base blocks
+                     dense_cols_scratch_bytes(d.dense_cols, (uint32_t)b) + 3 * (cols + 2) * 4;
     }
     return bytes + (16ull << 20);
 }

@@ -446,6 +446,162 @@ void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uin
     PGW_CHECK_LAUNCH("scale rows");
 }
 
+// ===========================================================================
+// deformation variants: dense columns beside the eliminated cyclic operator
+// (cyclic_dense_columns_*_kernel, cyclic_insert_complement_scaled_kernel,
+// cyclic_gather_complement_kernel: cuda/cyclic_cuda_helpers.h:1210-1351)
+// ===========================================================================
+
+// base position of kept column j (the base columns in order, drop0 < drop1 left out)
+__device__ __forceinline__ uint64_t kept_to_base(uint64_t j, uint32_t drop0, uint32_t drop1) {
+    if (j >= drop0) ++j;
+    if (j >= drop1) ++j;
+    return j;
+}
+
+__global__ __launch_bounds__(256) void k_complement_scale(const uint32_t* __restrict__ in,
+                                                          const uint32_t* __restrict__ dc,
+                                                          uint32_t* __restrict__ scaled, uint64_t base,
+                                                          uint32_t drop0, uint32_t drop1, uint64_t total,
+                                                          uint32_t b, ModP m) {
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * 256) {
+        const uint64_t r = e / b, c = e % b;
+        uint64_t src;  // input row feeding scaled row r
+        if (r < base) {
+            if (r == drop0 || r == drop1) {
+                scaled[e] = 0;
+                continue;
+            }
+            src = r - (r > drop0 ? 1 : 0) - (r > drop1 ? 1 : 0);
+        } else {
+            src = r - 2;  // dense-column inputs follow the base - 2 kept columns
+        }
+        scaled[e] = mulmod(in[src * b + c], dc[src], m);
+    }
+}
+
+void launch_complement_scale(const uint32_t* in, const uint32_t* dc, uint32_t* scaled,
+                             uint64_t base, uint32_t drop0, uint32_t drop1, uint32_t dcols,
+                             uint32_t b, const ModP& m, hipStream_t st) {
+    const uint64_t total = (base + dcols) * b;
+    if (total == 0) return;
+    unsigned g = (unsigned)std::min<uint64_t>(grid_for(total), 65536);
+    hipLaunchKernelGGL(k_complement_scale, dim3(g), dim3(256), 0, st, in, dc, scaled, base, drop0,
+                       drop1, total, b, m);
+    PGW_CHECK_LAUNCH("complement scale");
+}
+
+__global__ __launch_bounds__(256) void k_complement_gather(const uint32_t* __restrict__ full,
+                                                           uint32_t* __restrict__ out, uint64_t total,
+                                                           uint32_t drop0, uint32_t drop1, uint32_t b) {
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * 256) {
+        const uint64_t j = e / b, c = e % b;
+        out[e] = full[kept_to_base(j, drop0, drop1) * b + c];
+    }
+}
+
+void launch_complement_gather(const uint32_t* full, uint32_t* out, uint64_t base, uint32_t drop0,
+                              uint32_t drop1, uint32_t b, hipStream_t st) {
+    const uint64_t total = (base - 2) * b;
+    if (total == 0) return;
+    unsigned g = (unsigned)std::min<uint64_t>(grid_for(total), 65536);
+    hipLaunchKernelGGL(k_complement_gather, dim3(g), dim3(256), 0, st, full, out, total, drop0, drop1, b);
+    PGW_CHECK_LAUNCH("complement gather");
+}
+
+// one thread per target entry: sum_j dense[r][j] * extra[j][c], each product
+// reduced (any dcols), then added to or subtracted from the target entry
+__global__ __launch_bounds__(256) void k_dense_cols_axpy(uint32_t* __restrict__ target,
+                                                         const uint32_t* __restrict__ dense,
+                                                         const uint32_t* __restrict__ extra,
+                                                         uint64_t total, uint32_t dcols, uint32_t b,
+                                                         int subtract, ModP m) {
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * 256) {
+        const uint64_t r = e / b, c = e % b;
+        uint64_t sum = 0;  // dcols terms below 2^31 each
+        for (uint32_t j = 0; j < dcols; ++j)
+            sum += mulmod(dense[r * dcols + j], extra[(uint64_t)j * b + c], m);
+        const uint32_t t = mod_u64(sum, m);
+        const uint32_t x = target[e];
+        target[e] = subtract ? addmod(x, t ? m.p - t : 0u, m.p) : addmod(x, t, m.p);
+    }
+}
+
+void launch_dense_cols_axpy(uint32_t* target, const uint32_t* dense, const uint32_t* extra,
+                            uint64_t rows, uint32_t dcols, uint32_t b, bool subtract,
+                            const ModP& m, hipStream_t st) {
+    const uint64_t total = rows * b;
+    if (total == 0) return;
+    unsigned g = (unsigned)std::min<uint64_t>(grid_for(total), 65536);
+    hipLaunchKernelGGL(k_dense_cols_axpy, dim3(g), dim3(256), 0, st, target, dense, extra, total,
+                       dcols, b, subtract ? 1 : 0, m);
+    PGW_CHECK_LAUNCH("dense columns axpy");
+}
+
+// dense^T y in two passes and without atomics: each of at most kDenseTBlocks
+// workgroups sums its rows (a grid-stride walk) for every (j, c) pair with
+// reduced products in u64 (exact for any row count below 2^32), the second
+// pass adds the partials, reduces once, signs and scales.
+constexpr uint32_t kDenseTBlocks = 256;
+
+size_t dense_cols_scratch_bytes(uint32_t dcols, uint32_t b) {
+    return (size_t)kDenseTBlocks * dcols * b * sizeof(uint64_t);
+}
+
+__global__ __launch_bounds__(256) void k_dense_cols_t_partial(const uint32_t* __restrict__ dense,
+                                                              const uint32_t* __restrict__ y,
+                                                              uint64_t rows, uint32_t dcols, uint32_t b,
+                                                              uint64_t* __restrict__ partial, ModP m) {
+    __shared__ uint64_t s[256];
+    const uint32_t np = dcols * b;  // (j, c) pairs, np <= 256
+    const uint32_t rl = 256 / np;   // rows walked side by side
+    const uint32_t t = threadIdx.x, pair = t % np, lane = t / np;
+    uint64_t acc = 0;
+    if (lane < rl) {
+        const uint32_t j = pair / b, c = pair % b;
+        for (uint64_t r = (uint64_t)blockIdx.x * rl + lane; r < rows; r += (uint64_t)gridDim.x * rl)
+            acc += mulmod(dense[r * dcols + j], y[r * b + c], m);
+    }
+    s[t] = acc;
+    __syncthreads();
+    if (t < np) {
+        uint64_t sum = 0;
+        for (uint32_t q = 0; q < rl; ++q) sum += s[q * np + t];
+        partial[(uint64_t)blockIdx.x * np + t] = sum;
+    }
+}
+
+__global__ void k_dense_cols_t_finish(const uint64_t* __restrict__ partial, uint32_t blocks,
+                                      uint32_t dcols, uint32_t b, const uint32_t* __restrict__ dc,
+                                      uint64_t lead, uint32_t* __restrict__ out, int subtract, ModP m) {
+    const uint32_t np = dcols * b, t = threadIdx.x;
+    if (t >= np) return;
+    uint64_t sum = 0;  // blocks * (rows / blocks) * 2^31 < 2^63
+    for (uint32_t q = 0; q < blocks; ++q) sum += partial[(uint64_t)q * np + t];
+    uint32_t v = mod_u64(sum, m);
+    if (subtract && v) v = m.p - v;
+    const uint32_t j = t / b, c = t % b;
+    out[(lead + j) * b + c] = mulmod(v, dc[lead + j], m);
+}
+
+void launch_dense_cols_transpose(const uint32_t* dense, const uint32_t* y, uint64_t rows,
+                                 uint32_t dcols, uint32_t b, const uint32_t* dc, uint64_t lead,
+                                 uint32_t* out, bool subtract, uint64_t* scratch, const ModP& m,
+                                 hipStream_t st) {
+    const uint32_t np = dcols * b;
+    require(np >= 1 && np <= 256, "dense columns x block width must be at most 256");
+    const uint32_t rl = 256 / np;
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kDenseTBlocks, (rows + rl - 1) / rl));
+    hipLaunchKernelGGL(k_dense_cols_t_partial, dim3(blocks), dim3(256), 0, st, dense, y, rows, dcols,
+                       b, scratch, m);
+    hipLaunchKernelGGL(k_dense_cols_t_finish, dim3(1), dim3(256), 0, st, scratch, blocks, dcols, b,
+                       dc, lead, out, subtract ? 1 : 0, m);
+    PGW_CHECK_LAUNCH("dense columns transpose");
+}
+
 // v mod p for v < 2^63.  M31 (p = 2^31 - 1): 2^31 == 1, so two folds of the
 // high bits onto the low 31 bits leave v' < 2^31 + 2 and one conditional
 // subtraction finishes; otherwise 64-bit Barrett.

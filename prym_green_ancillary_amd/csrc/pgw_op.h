@@ -125,6 +125,14 @@ struct pgw_op {
     uint32_t* d_y = nullptr;
     uint32_t* d_h = nullptr;
     uint32_t* d_tadj = nullptr;
+    // deformation variants (augmented / replacement) of the cyclic operator
+    uint64_t base_cols = 0;              // columns of the eliminated base operator
+    uint32_t dense_cols = 0, drop0 = 0, drop1 = 0;
+    uint32_t* d_dense = nullptr;         // rows x dense_cols
+    uint32_t* d_base_dc = nullptr;       // Dc at the base positions (replacement: 0 at the drops)
+    uint32_t* d_base_negdc = nullptr;    // p - d_base_dc
+    uint32_t* d_base_adj = nullptr;      // replacement: F^T of the target, base_cols x b
+    uint64_t* d_dense_scratch = nullptr; // partial sums of dense^T (target)
     // timing of the sweep kernels (pgw_bench_apply)
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> sweep_events;

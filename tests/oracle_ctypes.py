@@ -71,6 +71,8 @@ def lib():
     L.orc_cyclic_create.restype = _c.c_void_p
     L.orc_cyclic_create.argtypes = [_c.c_uint32] * 4 + [_u32p, _u16p, _u16p, _u16p, _c.c_uint32,
                                                         _c.c_uint32, _c.c_uint32, _u16p, _u32p, _u32p]
+    L.orc_cyclic_deformation.restype = _c.c_int
+    L.orc_cyclic_deformation.argtypes = [_c.c_void_p, _c.c_int, _u32p, sz, sz, sz]
     L.orc_set_preconditioners.restype = None
     L.orc_set_preconditioners.argtypes = [_c.c_void_p, _u32p, _u32p, sz, _c.c_void_p]
     L.orc_op_dims.restype = None
@@ -119,6 +121,21 @@ class OracleOp:
         rows, cols = ctypes.c_size_t(), ctypes.c_size_t()
         lib().orc_op_dims(self.h, ctypes.byref(rows), ctypes.byref(cols))
         self.rows, self.cols = rows.value, cols.value
+
+    def deform(self, variant: str, dense, drops=(0, 0)):
+        """Turn an eliminated cyclic operator into its augmented [F0 | -dense]
+        or replacement [F0 without the two drop columns | dense] variant."""
+        dense = u32(dense)
+        assert dense.ndim == 2 and dense.shape[0] == self.rows
+        code = {"augmented": 1, "replacement": 2}[variant]
+        self._keep.append(dense)
+        rc = lib().orc_cyclic_deformation(self.h, code, dense.reshape(-1), dense.shape[1],
+                                          drops[0], drops[1])
+        assert rc == 0, "invalid deformation request"
+        rows, cols = ctypes.c_size_t(), ctypes.c_size_t()
+        lib().orc_op_dims(self.h, ctypes.byref(rows), ctypes.byref(cols))
+        self.rows, self.cols = rows.value, cols.value
+        return self
 
     def set_preconditioners(self, dr, dc, rounds=0, pairs=None):
         dr, dc = u32(dr), u32(dc)
