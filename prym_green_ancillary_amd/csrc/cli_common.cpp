@@ -2,6 +2,7 @@
 
 #include "cli_common.h"
 
+#include <algorithm>
 #include <cctype>
 #include <filesystem>
 #include <fstream>
@@ -176,6 +177,29 @@ Instance load_instance(const std::string& path) {
         I.expected_sector_rows = integers(array(e, "sector_rows", false));
     }
     return I;
+}
+
+DenseBlock load_dense_block(const std::string& path, uint32_t prime) {
+    const std::string text = read_text(path);
+    DenseBlock D;
+    D.rows = number(text, "rows");
+    D.cols = number(text, "columns");
+    if (number(text, "prime") != prime) throw std::runtime_error("dense block prime does not match instance modulus");
+    std::vector<uint64_t> raw = integers(array(text, "values"));
+    if (raw.size() != D.rows * D.cols) throw std::runtime_error("dense block has the wrong number of values");
+    D.values = narrow<uint32_t>(raw, prime, "dense block");
+    return D;
+}
+
+std::vector<uint32_t> load_drop_columns(const std::string& path, size_t expected, uint64_t base_cols) {
+    const std::string text = read_text(path);
+    std::vector<uint64_t> raw = integers(array(text, "indices"));
+    if (raw.size() != expected) throw std::runtime_error("drop-column file has the wrong number of indices");
+    std::vector<uint32_t> out = narrow<uint32_t>(raw, base_cols, "drop-column index (outside the base column range)");
+    std::sort(out.begin(), out.end());
+    for (size_t i = 1; i < out.size(); ++i)
+        if (out[i] == out[i - 1]) throw std::runtime_error("drop-column indices must be distinct");
+    return out;
 }
 
 pgw_operator_desc make_desc(Instance& I, uint32_t sector, uint32_t block, uint32_t method) {

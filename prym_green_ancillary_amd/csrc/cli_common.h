@@ -33,6 +33,17 @@ struct Instance {
 };
 
 Instance load_instance(const std::string& path);
+
+// Dense deformation block file {"rows", "columns", "prime", "values"}, values
+// row-major and reduced (load_dense_block_file, cuda/cuprym_deformation.cu:426-450).
+struct DenseBlock {
+    uint64_t rows = 0, cols = 0;
+    std::vector<uint32_t> values;
+};
+DenseBlock load_dense_block(const std::string& path, uint32_t prime);
+// Drop-column file {"indices": [...]}: `expected` distinct indices below
+// `base_cols`, returned ascending (load_drop_columns_file, :452-472).
+std::vector<uint32_t> load_drop_columns(const std::string& path, size_t expected, uint64_t base_cols);
 // Fills a pgw_operator_desc pointing into `inst` (which must outlive it).
 pgw_operator_desc make_desc(Instance& inst, uint32_t sector, uint32_t block, uint32_t method);
 

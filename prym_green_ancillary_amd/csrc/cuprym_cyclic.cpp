@@ -1,7 +1,8 @@
 // cuprym_cyclic -- sector sequence generator (the reference's cuprym_cyclic
 // executable, cuda/cuprym_cyclic.cu:2498-2689), driving libpgw through its C ABI.
 //
-//   cuprym_cyclic INSTANCE --sector S [--operator eliminated|synthetic] [--out DIR]
+//   cuprym_cyclic INSTANCE --sector S [--operator eliminated|augmented|replacement|synthetic]
+//       [--dense-file F] [--drop-columns F] [--out DIR]
 //       [-d DEV] [-v B] [-s 0] [--seed 1] [--rowmix-rounds 2] [--rowmix-seed 10001]
 //       [--validate-rowmix] [--length L] [--extra-terms 50] [--method auto|plain|mfma]
 //       [--no-save]
@@ -72,11 +73,13 @@ int main(int argc, char** argv) {
             {{"--seed"}, false}, {{"--rowmix-rounds"}, false}, {{"--rowmix-seed"}, false},
             {{"--validate-rowmix"}, true}, {{"--length"}, false}, {{"--extra-terms"}, false},
             {{"--method"}, false}, {{"--no-save"}, true}, {{"-h", "--help"}, true},
+            {{"--dense-file"}, false}, {{"--drop-columns"}, false},
         };
         Args a = parse_args(argc, argv, specs);
         if (has(a, "--help") || a.positional.size() != 1 || !has(a, "--sector")) {
-            std::cerr << "usage: cuprym_cyclic INSTANCE --sector S [--operator eliminated|synthetic] "
-                         "[--out DIR] [-d DEV] [-v B] [--seed N] [--rowmix-rounds R] "
+            std::cerr << "usage: cuprym_cyclic INSTANCE --sector S "
+                         "[--operator eliminated|augmented|replacement|synthetic] [--dense-file F] "
+                         "[--drop-columns F] [--out DIR] [-d DEV] [-v B] [--seed N] [--rowmix-rounds R] "
                          "[--rowmix-seed N] [--validate-rowmix] [--length L] [--extra-terms E] "
                          "[--method auto|plain|mfma] [--no-save]\n";
             return has(a, "--help") ? 0 : 1;
@@ -102,11 +105,18 @@ int main(int argc, char** argv) {
         std::cout << "Loading instance JSON: " << instance_path << std::endl;
         Instance inst = load_instance(instance_path);
         const std::string op_name = get(a, "--operator", inst.synthetic ? "synthetic" : "eliminated");
-        if (op_name != (inst.synthetic ? "synthetic" : "eliminated")) {
-            std::cerr << "--operator must be " << (inst.synthetic ? "synthetic" : "eliminated")
+        // the deformation variants (cuda/cuprym_deformation.cu:3091-3123, 2578-2623)
+        const bool deform = op_name == "augmented" || op_name == "replacement";
+        if (inst.synthetic ? op_name != "synthetic" : (op_name != "eliminated" && !deform)) {
+            std::cerr << "--operator must be " << (inst.synthetic ? "synthetic" : "one of: eliminated, augmented, replacement")
                       << " for this instance." << std::endl;
             return 1;
         }
+        if (deform && !has(a, "--dense-file")) throw std::runtime_error("deformation operators require --dense-file");
+        if (op_name == "replacement" && !has(a, "--drop-columns"))
+            throw std::runtime_error("replacement operator requires --drop-columns");
+        if (op_name != "replacement" && has(a, "--drop-columns"))
+            throw std::runtime_error("--drop-columns is only meaningful for replacement");
         if (!is_prime(inst.modulus)) {
             std::cerr << "Modulus " << inst.modulus << " is not prime." << std::endl;
             return 1;
@@ -123,6 +133,23 @@ int main(int argc, char** argv) {
         if (!inst.synthetic && sector < inst.expected_sector_columns.size() &&
             (inst.expected_sector_columns[sector] != cols || inst.expected_sector_rows[sector] != rows))
             throw std::runtime_error("eliminated sector dimensions do not match fixture metadata");
+        DenseBlock dense;
+        if (deform) {
+            dense = load_dense_block(get(a, "--dense-file", ""), inst.modulus);
+            if (dense.rows != rows || dense.cols != 2)
+                throw std::runtime_error("dense block dimensions do not match the eliminated target sector");
+            desc.kind = op_name == "augmented" ? PGW_KIND_CYCLIC_AUGMENTED : PGW_KIND_CYCLIC_REPLACEMENT;
+            desc.dense_cols = (uint32_t)dense.cols;
+            desc.dense_block = dense.values.data();
+            if (op_name == "replacement") {
+                const std::vector<uint32_t> drops = load_drop_columns(get(a, "--drop-columns", ""), dense.cols, cols);
+                desc.drop_cols[0] = drops[0];
+                desc.drop_cols[1] = drops[1];
+                std::cout << "Dropped base columns: " << drops[0] << " " << drops[1] << std::endl;
+            }
+            std::cout << "Base F0 dimensions: " << rows << " x " << cols << std::endl;
+            check(pgw_operator_dims(&desc, &rows, &cols));
+        }
         std::cout << "Sector " << sector << ": " << rows << " x " << cols << " (" << op_name
                   << ", p=" << inst.modulus << ", b=" << num_v << ")" << std::endl;
 

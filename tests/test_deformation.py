@@ -209,3 +209,79 @@ def test_gpu_augmented_kernel_certificate(g12, g12_reference, gctx, sector):
     op.apply_forward(x.data_ptr(), t.data_ptr(), torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     assert not t.cpu().numpy().any()
+
+
+# --------------------------------------------------------------------- CLI --
+
+def _write_inputs(tmp_path, p, dense, drops=None):
+    dense = np.asarray(dense, dtype=np.int64)
+    df = tmp_path / "dense.json"
+    df.write_text(json_dumps({"rows": dense.shape[0], "columns": dense.shape[1], "prime": p,
+                              "values": dense.tolist()}))
+    if drops is None:
+        return df, None
+    dc = tmp_path / "drop.json"
+    dc.write_text(json_dumps({"indices": list(drops)}))
+    return df, dc
+
+
+def json_dumps(obj):
+    import json
+    return json.dumps(obj)
+
+
+def _cli(args):
+    import subprocess
+    gen = oc.ROOT / "prym_green_ancillary_amd" / "bin" / "cuprym_cyclic"
+    return subprocess.run([str(gen)] + [str(a) for a in args], capture_output=True, text=True, timeout=600)
+
+
+G12 = oc.ROOT / "tests" / "golden" / "g12.json"
+
+
+def test_generator_cli_refuses_bad_deformation_inputs(tmp_path, g12, g12_reference):
+    p = g12["modulus"]
+    Z0, J = replacement_parts(g12_reference)
+    df, dcf = _write_inputs(tmp_path, p, Z0, J)
+    cases = [
+        (["--operator", "replacement", "--dense-file", df], "requires --drop-columns"),
+        (["--operator", "augmented", "--dense-file", df, "--drop-columns", dcf], "only meaningful for replacement"),
+        (["--operator", "augmented", "--dense-file", df, "--sector", 1], "dimensions do not match"),
+    ]
+    bad_drop = tmp_path / "bad.json"
+    bad_drop.write_text('{"indices": [3, 3]}')
+    cases.append((["--operator", "replacement", "--dense-file", df, "--drop-columns", bad_drop], "distinct"))
+    wrong_p = tmp_path / "wrongp.json"
+    wrong_p.write_text(json_dumps({"rows": Z0.shape[0], "columns": 2, "prime": 661, "values": Z0.tolist()}))
+    cases.append((["--operator", "augmented", "--dense-file", wrong_p], "prime does not match"))
+    for extra, msg in cases:
+        args = [G12, "--no-save", "--out", tmp_path] + extra
+        if "--sector" not in extra:
+            args += ["--sector", 0]
+        r = _cli(args)
+        assert r.returncode == 1 and msg in (r.stderr + r.stdout), (extra, r.stderr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,sector", [("augmented", 1), ("replacement", 0)])
+def test_generator_cli_deformation_writes_oracle_wdm(tmp_path, g12, g12_reference, variant, sector):
+    import test_cli as tc
+    p, b = g12["modulus"], 4
+    dense, drops = _variant_setup(g12, g12_reference, variant, sector)
+    df, dcf = _write_inputs(tmp_path, p, dense, drops if variant == "replacement" else None)
+    args = [G12, "--sector", sector, "--out", tmp_path, "-v", b, "--operator", variant, "--dense-file", df]
+    if dcf is not None:
+        args += ["--drop-columns", dcf]
+    r = _cli(args)
+    assert r.returncode == 0, r.stderr
+    name = f"{variant}-sector{sector:02d}-v{b}-rowmix-r2-s10001"
+    text = tc.decompress(tmp_path / f"{name}.wdm.zst")
+    o = oc.cyclic_op(g12, sector).deform(variant, dense, drops)
+    dr, dc, v = oc.reference_random_state(1, sector, p, o.rows, o.cols, b)
+    _, pairs, _ = oc.rowmix_plan(10001, variant, sector, o.rows, p, 2)
+    o.set_preconditioners(dr, dc, 2, pairs)
+    seq, curv = o.sequence(v, b, 2 * o.cols // b + 50)
+    ref = tmp_path / "oracle.wdm"
+    oc.lib().orc_write_wdm(str(ref).encode(), o.rows, o.cols, p, dr, dc, v, curv, b,
+                           np.ascontiguousarray(seq.reshape(-1)), seq.shape[0])
+    assert text == ref.read_text()
