@@ -20,7 +20,11 @@
  *   - Gram / sequence: closed form V^T S^t V against dense powers;
  *   - WDM text: the hand-made diag(1,2) file of the BCW CLI test;
  *   - row-mix plan: structure (disjoint pairs, coefficient range) and the
- *     transpose identity only.
+ *     transpose identity only;
+ *   - deformation variants: the g12 certificate (tests/test_deformation.py):
+ *     [S_c; I] in the kernel of [F0_c | -F0_c S_c] with rank 86 - 2 = 84
+ *     (augmented_shape / augmented_nullity), rank 84 of [F0_0 without J | Z0]
+ *     (completion_rank), transpose identities.
  * Parity unpinned: the exact row-mix pair lists / plan hashes and sidecar
  * hash strings of a real reference run, and the reference binary's own WDM
  * bytes -- no reference output exists to compare with and the CUDA sources
