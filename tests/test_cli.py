@@ -62,7 +62,7 @@ def decompress(path: Path) -> str:
     (["--sector", "0", "--method", "fast"], "--method must be"),
     (["--sector", "0", "-v", "17"], "-v must be in 1..16"),
     (["--sector", "0", "--bogus", "1"], "unknown option"),
-    (["--sector", "0", "--operator", "synthetic"], "--operator must be eliminated"),
+    (["--sector", "0", "--operator", "synthetic"], "--operator must be one of: eliminated, augmented, replacement"),
 ])
 def test_generator_cli_rejects_bad_options(args, msg):
     assert GEN.exists(), "build the drivers first (make -C prym_green_ancillary_amd/csrc)"
