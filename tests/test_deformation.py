@@ -285,3 +285,30 @@ def test_generator_cli_deformation_writes_oracle_wdm(tmp_path, g12, g12_referenc
     oc.lib().orc_write_wdm(str(ref).encode(), o.rows, o.cols, p, dr, dc, v, curv, b,
                            np.ascontiguousarray(seq.reshape(-1)), seq.shape[0])
     assert text == ref.read_text()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,sector,rank", [
+    ("augmented", 1, 84), ("replacement", 0, 84),
+    # over F_109 the reference's fixed preconditioners lose one Gram rank in
+    # augmented sector 5 (tests/check_cuda.py:71-73); the same recipe here
+    # reproduces that loss, the exact rank (86 - 2) being pinned above
+    ("augmented", 5, 83)])
+def test_gpu_deformation_sequence_rank(g12, g12_reference, gctx, variant, sector, rank):
+    import prym_green_ancillary_amd as pg
+    p, b = g12["modulus"], 4
+    print(f"deformation rank {variant} sector {sector}", flush=True)
+    o, op, dr, dc, v = _pair(g12, g12_reference, gctx, variant, sector, b)
+    seq, _ = op.sequence(v, 2 * o.cols // b + 50)
+    r = pg.RankComputation(p, b, o.rows, o.cols)
+    st = r.step(seq)
+    assert st.status == pg.RANK_SUCCESS and st.rank_lower_bound == rank
+    _, rank_o, spread_o, gen_o = oc.bcw_rank(p, b, seq)
+    assert (rank_o, spread_o) == (st.rank_lower_bound, st.delta_spread)
+    assert np.array_equal(r.generator(), gen_o)
+    # the generator's kernel vectors are annihilated by G (the certificate)
+    # exactly when no Gram rank was lost; a lost rank leaves a vector in the
+    # kernel of Dc G^T P^T Dr P G Dc that G does not annihilate
+    d, prod = op.kernel_vectors(v, r.generator().reshape(-1, b * b))
+    exact = o.cols - 2 if variant == "augmented" else g12_reference["completion_rank"]
+    assert bool(prod.any()) == (rank != exact)
