@@ -138,7 +138,7 @@ void launch_dense_mfma_b16(int, uint32_t, const DenseSweep&, const MfmaTable&, c
 void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
                        const uint32_t* scale, uint32_t b, uint32_t nl, const ModP& m,
                        hipStream_t st, int cfg) {
-    if (cfg < 0 || cfg >= mfma_config_count()) cfg = 1;
+    if (cfg < 0 || cfg >= mfma_config_count()) cfg = 8;
     if (b == 16) launch_dense_mfma_b16(cfg, nl, s, t, lp, out, scale, m, st);
     else if (b == 8) launch_dense_mfma_b8(cfg, nl, s, t, lp, out, scale, m, st);
     else if (b == 4) launch_dense_mfma_b4(cfg, nl, s, t, lp, out, scale, m, st);

@@ -108,7 +108,7 @@ struct pgw_op {
     uint32_t* d_tab[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     uint32_t k4[4] = {0, 0, 0, 0};
     MfmaTable mt[4];
-    int mfma_cfg[4] = {1, 1, 1, 1};  // tile configuration per sweep (pgw_operator_autotune)
+    int mfma_cfg[4] = {8, 8, 8, 8};  // tile configuration per sweep (pgw_operator_autotune)
     uint32_t nl = 0;
     int8_t* d_lp_src = nullptr;
     int8_t* d_lp_mid = nullptr;
