@@ -373,6 +373,22 @@ int pgw_server_enable_rank(pgw_server* s, uint32_t threads, uint32_t threshold,
 int pgw_server_rank(pgw_server* s, uint64_t ticket, pgw_rank_info* info, uint32_t* h_gen,
                     uint64_t gen_capacity, uint64_t* gen_terms, double* seconds);
 
+/* ---- WDM output ------------------------------------------------------------ */
+
+/* save_all_data + save_wdm_sym (cuda/cuprym_cyclic.cu:965-1007,
+ * cuda/wdmfiles.h:164-222, zstd framing cuda/zstd_compat.h:210-333): writes
+ * the symmetric WDM text of one sector via path + ".tmp" and a rename, as one
+ * zstd frame (level 3) when path ends in .zst.  h_v, h_curv: cols x block
+ * row-major; h_seq: length x block(block+1)/2 row-major.  The text is
+ * formatted on `threads` host threads in ordered pieces and compressed by
+ * libzstd's worker threads (threads = 0: PGW_WDM_THREADS or the host's
+ * hardware threads, at most 16); the text is the same for every thread count.
+ * Errors: PGW_ERR_IO. */
+int pgw_wdm_write(const char* path, uint64_t rows, uint64_t cols, uint32_t prime,
+                  const uint32_t* h_dr, const uint32_t* h_dc, const uint32_t* h_v,
+                  const uint32_t* h_curv, uint32_t block, const uint32_t* h_seq, uint64_t length,
+                  uint32_t threads);
+
 
 
 #ifdef __cplusplus

@@ -104,7 +104,7 @@ EXPORTED = [
     "pgw_server_submit", "pgw_server_wait", "pgw_server_result", "pgw_server_stats",
     "pgw_server_destroy", "pgw_job_footprint", "pgw_rank_create", "pgw_rank_step",
     "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
-    "pgw_server_enable_rank", "pgw_server_rank",
+    "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write",
 ]
 
 
@@ -178,6 +178,7 @@ def lib():
         "pgw_server_enable_rank": (I, [vp, u32, u32, u32, I]),
         "pgw_server_rank": (I, [vp, u64, ctypes.POINTER(RankInfo), vp, u64, ctypes.POINTER(u64),
                                 ctypes.POINTER(ctypes.c_double)]),
+        "pgw_wdm_write": (I, [ctypes.c_char_p, u64, u64, u32, vp, vp, vp, vp, u32, vp, u64, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -534,6 +535,19 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def write_wdm(path, rows: int, cols: int, p: int, dr, dc, v, curv, b: int, seq, threads: int = 0):
+    """Symmetric WDM file (one zstd frame for .zst paths) via pgw_wdm_write;
+    seq is (L, b(b+1)/2), v and curv (cols, b)."""
+    dr, dc = _u32(dr).reshape(-1), _u32(dc).reshape(-1)
+    v, curv, seq = _u32(v).reshape(-1), _u32(curv).reshape(-1), _u32(seq)
+    L = seq.shape[0] if seq.ndim == 2 else seq.size // (b * (b + 1) // 2)
+    if dr.size != rows or dc.size != cols or v.size != cols * b or curv.size != cols * b or \
+            seq.size != L * (b * (b + 1) // 2):
+        raise ValueError("WDM array shapes do not match rows, cols and b")
+    _check(lib().pgw_wdm_write(str(path).encode(), rows, cols, p, _ptr(dr), _ptr(dc), _ptr(v),
+                               _ptr(curv), b, _ptr(seq.reshape(-1)), L, threads))
 
 
 def job_footprint(desc: OperatorDesc, max_len: int, rounds: int = 2) -> int:

@@ -2,13 +2,18 @@
 
 #include "wdm.h"
 
+#include <algorithm>
 #include <charconv>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
 
 // libzstd's stable streaming API (zstd.h is not part of this image, the
 // shared library libzstd.so.1 is): opaque stream types and buffer structs.
@@ -33,6 +38,12 @@ size_t ZSTD_initCStream(ZSTD_CStream*, int level);
 size_t ZSTD_compressStream(ZSTD_CStream*, ZSTD_outBuffer*, ZSTD_inBuffer*);
 size_t ZSTD_endStream(ZSTD_CStream*, ZSTD_outBuffer*);
 size_t ZSTD_CStreamOutSize(void);
+// advanced one-frame streaming with worker threads (stable since zstd 1.4.0):
+// parameter ids and end directives as libzstd defines them
+ZSTD_CStream* ZSTD_createCCtx(void);
+size_t ZSTD_freeCCtx(ZSTD_CStream*);
+size_t ZSTD_CCtx_setParameter(ZSTD_CStream*, int param, int value);
+size_t ZSTD_compressStream2(ZSTD_CStream*, ZSTD_outBuffer*, ZSTD_inBuffer*, int end_op);
 ZSTD_DStream* ZSTD_createDStream(void);
 size_t ZSTD_freeDStream(ZSTD_DStream*);
 size_t ZSTD_initDStream(ZSTD_DStream*);
@@ -43,6 +54,21 @@ const char* ZSTD_getErrorName(size_t);
 }
 
 namespace pgwio {
+
+namespace {
+constexpr int kZstdCompressionLevel = 100;  // ZSTD_c_compressionLevel
+constexpr int kZstdNbWorkers = 400;         // ZSTD_c_nbWorkers
+constexpr int kZstdContinue = 0, kZstdEnd = 2;
+}  // namespace
+
+unsigned writer_threads() {
+    if (const char* env = std::getenv("PGW_WDM_THREADS")) {
+        const long t = std::strtol(env, nullptr, 10);
+        if (t >= 1 && t <= 256) return (unsigned)t;
+    }
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw == 0 ? 1u : (hw > 16 ? 16u : hw);
+}
 
 static bool ends_with(const std::string& s, const std::string& suf) {
     return s.size() >= suf.size() && s.compare(s.size() - suf.size(), suf.size(), suf) == 0;
@@ -55,7 +81,7 @@ bool path_requests_compression(const std::string& path) {
 struct TextSink::Impl {
     FILE* f = nullptr;
     bool compress = false;
-    ZSTD_CStream* zs = nullptr;
+    ZSTD_CStream* zs = nullptr;  // a ZSTD_CCtx: one frame, level 3, worker threads when libzstd has them
     std::vector<char> buf, zout;
     size_t used = 0;
     std::string path;
@@ -72,7 +98,7 @@ struct TextSink::Impl {
             ZSTD_inBuffer in{buf.data(), used, 0};
             while (in.pos < in.size) {
                 ZSTD_outBuffer out{zout.data(), zout.size(), 0};
-                size_t rc = ZSTD_compressStream(zs, &out, &in);
+                size_t rc = ZSTD_compressStream2(zs, &out, &in, kZstdContinue);
                 if (ZSTD_isError(rc)) throw std::runtime_error(std::string("zstd: ") + ZSTD_getErrorName(rc));
                 write_raw(zout.data(), out.pos);
             }
@@ -81,23 +107,26 @@ struct TextSink::Impl {
     }
 };
 
-TextSink::TextSink(const std::string& path, bool compress) : impl_(new Impl) {
+TextSink::TextSink(const std::string& path, bool compress, unsigned threads) : impl_(new Impl) {
     impl_->path = path;
     impl_->compress = compress;
     impl_->f = std::fopen(path.c_str(), "wb");
     if (!impl_->f) throw std::runtime_error("failed to open " + path);
     impl_->buf.resize(1 << 22);
     if (compress) {
-        impl_->zs = ZSTD_createCStream();
+        impl_->zs = ZSTD_createCCtx();
         if (!impl_->zs) throw std::runtime_error("failed to allocate a zstd stream");
-        size_t rc = ZSTD_initCStream(impl_->zs, 3);
+        size_t rc = ZSTD_CCtx_setParameter(impl_->zs, kZstdCompressionLevel, 3);
         if (ZSTD_isError(rc)) throw std::runtime_error(std::string("zstd init: ") + ZSTD_getErrorName(rc));
+        // worker threads compress blocks of the same frame; a libzstd built
+        // without them refuses the parameter and compresses on this thread
+        if (threads > 1) (void)ZSTD_CCtx_setParameter(impl_->zs, kZstdNbWorkers, (int)threads);
         impl_->zout.resize(ZSTD_CStreamOutSize());
     }
 }
 
 TextSink::~TextSink() {
-    if (impl_->zs) ZSTD_freeCStream(impl_->zs);
+    if (impl_->zs) ZSTD_freeCCtx(impl_->zs);
     if (impl_->f) std::fclose(impl_->f);
 }
 
@@ -126,7 +155,8 @@ void TextSink::finish() {
     if (impl_->compress) {
         for (;;) {
             ZSTD_outBuffer out{impl_->zout.data(), impl_->zout.size(), 0};
-            size_t left = ZSTD_endStream(impl_->zs, &out);
+            ZSTD_inBuffer none{nullptr, 0, 0};
+            size_t left = ZSTD_compressStream2(impl_->zs, &out, &none, kZstdEnd);
             if (ZSTD_isError(left)) throw std::runtime_error(std::string("zstd end: ") + ZSTD_getErrorName(left));
             impl_->write_raw(impl_->zout.data(), out.pos);
             if (left == 0) break;
@@ -173,36 +203,86 @@ std::string read_maybe_compressed(const std::string& path) {
     return text;
 }
 
-static void put_line_strided(TextSink& s, const uint32_t* v, uint64_t count, uint64_t stride,
-                             uint32_t p) {
-    for (uint64_t i = 0; i < count; ++i) {
-        if (i) s.put_char(' ');
-        s.put_u64(v[i * stride] % p);
+// values i in [begin, end) of a line, each preceded by a space except the line's first
+static void format_range(const uint32_t* v, uint64_t begin, uint64_t end, uint64_t stride, uint32_t p,
+                         std::string& out) {
+    out.clear();
+    out.reserve((end - begin) * 11);
+    char tmp[16];
+    for (uint64_t i = begin; i < end; ++i) {
+        if (i) out.push_back(' ');
+        auto r = std::to_chars(tmp, tmp + sizeof(tmp), p ? v[i * stride] % p : v[i * stride]);
+        out.append(tmp, (size_t)(r.ptr - tmp));
+    }
+}
+
+// One text line of `count` values (p = 0: written as stored).  Long lines are
+// formatted in pieces, `threads` pieces (a window) at a time on worker
+// threads, while this thread hands the previous window to the sink (zstd
+// compresses on the calling thread): pieces go out in order, so the bytes are
+// those of a serial writer.
+static void put_line(TextSink& s, const uint32_t* v, uint64_t count, uint64_t stride, uint32_t p,
+                     unsigned threads) {
+    constexpr uint64_t kPiece = 1ull << 18;  // values per piece (about 2.8 MB of text)
+    const uint64_t window = kPiece * threads;
+    if (threads <= 1 || count <= kPiece) {
+        std::string piece;
+        for (uint64_t b = 0; b < count; b += kPiece) {
+            format_range(v, b, std::min(count, b + kPiece), stride, p, piece);
+            s.put(piece.data(), piece.size());
+        }
+        s.put_char('\n');
+        return;
+    }
+    std::vector<std::string> bufs[2] = {std::vector<std::string>(threads), std::vector<std::string>(threads)};
+    std::vector<std::thread> pool;
+    auto launch = [&](uint64_t w0, std::vector<std::string>& dst) {  // format window w0 in the background
+        for (unsigned q = 0; q < threads; ++q) {
+            const uint64_t b = w0 + q * kPiece;
+            if (b >= count) break;
+            pool.emplace_back([&, b, q] { format_range(v, b, std::min(count, b + kPiece), stride, p, dst[q]); });
+        }
+    };
+    auto join = [&] {
+        for (auto& t : pool) t.join();
+        pool.clear();
+    };
+    launch(0, bufs[0]);
+    join();
+    for (uint64_t w0 = 0, k = 0; w0 < count; w0 += window, k ^= 1) {
+        if (w0 + window < count) launch(w0 + window, bufs[k ^ 1]);
+        try {
+            for (unsigned q = 0; q < threads && w0 + q * kPiece < count; ++q)
+                s.put(bufs[k][q].data(), bufs[k][q].size());
+        } catch (...) {
+            join();
+            throw;
+        }
+        join();
     }
     s.put_char('\n');
 }
 
 void save_wdm_sym(const std::string& path, uint64_t rows, uint64_t cols, uint32_t p,
                   const uint32_t* dr, const uint32_t* dc, const uint32_t* v, const uint32_t* curv,
-                  uint32_t b, const uint32_t* seq, uint64_t L) {
+                  uint32_t b, const uint32_t* seq, uint64_t L, unsigned threads) {
     if (L == 0) throw std::runtime_error("Cannot save WDM file with empty sequence list");
+    if (threads == 0) threads = writer_threads();
     const std::string tmp = path + ".tmp";
     {
-        TextSink s(tmp, path_requests_compression(tmp));
+        TextSink s(tmp, path_requests_compression(tmp), threads);
         s.put_u64(rows); s.put_char(' ');
         s.put_u64(cols); s.put_char(' ');
         s.put_u64(p); s.put_char(' ');
         s.put_u64(L); s.put_char(' ');
         s.put_u64(b); s.put_char('\n');
         // preconditioners are written as stored (cuda/wdmfiles.h:195-199)
-        for (uint64_t i = 0; i < rows; ++i) { if (i) s.put_char(' '); s.put_u64(dr[i]); }
-        s.put_char('\n');
-        for (uint64_t i = 0; i < cols; ++i) { if (i) s.put_char(' '); s.put_u64(dc[i]); }
-        s.put_char('\n');
-        for (uint32_t c = 0; c < b; ++c) put_line_strided(s, v + c, cols, b, p);
-        for (uint32_t c = 0; c < b; ++c) put_line_strided(s, curv + c, cols, b, p);
+        put_line(s, dr, rows, 1, 0, threads);
+        put_line(s, dc, cols, 1, 0, threads);
+        for (uint32_t c = 0; c < b; ++c) put_line(s, v + c, cols, b, p, threads);
+        for (uint32_t c = 0; c < b; ++c) put_line(s, curv + c, cols, b, p, threads);
         const uint64_t ut = (uint64_t)b * (b + 1) / 2;
-        for (uint64_t e = 0; e < ut; ++e) put_line_strided(s, seq + e, L, ut, p);
+        for (uint64_t e = 0; e < ut; ++e) put_line(s, seq + e, L, ut, p, threads);
         s.finish();
     }
     std::filesystem::rename(tmp, path);

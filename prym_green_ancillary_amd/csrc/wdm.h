@@ -17,10 +17,15 @@ namespace pgwio {
 
 bool path_requests_compression(const std::string& path);
 
-// Buffered text sink writing plain bytes or one zstd frame.
+// Threads of the WDM writer: PGW_WDM_THREADS, else the host's hardware
+// threads, at most 16.
+unsigned writer_threads();
+
+// Buffered text sink writing plain bytes or one zstd frame (level 3; with
+// threads > 1, libzstd's worker threads compress blocks of that frame).
 class TextSink {
 public:
-    TextSink(const std::string& path, bool compress);
+    TextSink(const std::string& path, bool compress, unsigned threads = 1);
     ~TextSink();
     void put(const char* data, size_t n);
     void put_u64(uint64_t v);
@@ -37,10 +42,13 @@ std::string read_maybe_compressed(const std::string& path);
 
 // Writes `path` via path + ".tmp" and a rename, as save_all_data does
 // (cuda/cuprym_cyclic.cu:965-1007).  v and curv are row-major cols x b,
-// seq is L x b(b+1)/2 row-major (row t = packed V^T S^t V).
+// seq is L x b(b+1)/2 row-major (row t = packed V^T S^t V).  The text is
+// formatted on `threads` threads (0: writer_threads()) in pieces that are
+// written in order, so every thread count gives the same text (and one
+// zstd frame for .zst paths).
 void save_wdm_sym(const std::string& path, uint64_t rows, uint64_t cols, uint32_t p,
                   const uint32_t* dr, const uint32_t* dc, const uint32_t* v, const uint32_t* curv,
-                  uint32_t b, const uint32_t* seq, uint64_t L);
+                  uint32_t b, const uint32_t* seq, uint64_t L, unsigned threads = 0);
 
 struct WdmInitial {
     uint64_t rows = 0, cols = 0, length = 0;
