@@ -60,6 +60,7 @@ DenseSweep sweep_desc(const pgw_op* op, int which) {
     s.r_out = s.fwd ? op->Mp : op->M;
     s.k_in = s.fwd ? op->M : op->Mp;
     s.binom = op->ctx->d_binom;
+    s.inc = op->d_inc[s.fwd ? 0 : 1];
     return s;
 }
 
@@ -373,6 +374,11 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
                     const int c = std::atoi(env);
                     require(c >= 0 && c < mfma_config_count(), "PGW_MFMA_CFG out of range");
                     for (int& x : op->mfma_cfg) x = c;
+                }
+                for (int w = 0; w < 2; ++w) {  // sweep 0 is forward, sweep 1 transpose
+                    const DenseSweep s = sweep_desc(op, w);
+                    op->d_inc[w] = A.upload(mfma_incidence_table(op->n, s.fwd ? op->k - 1 : op->k, s.fwd,
+                                                                 s.u_out, s.v_in));
                 }
                 for (int w = 0; w < 4; ++w) {
                     DenseSweep s = sweep_desc(op, w);

@@ -471,6 +471,7 @@ uint64_t estimate_job_bytes(const pgw_operator_desc& d, uint64_t rows, uint64_t 
         bytes += 4 * (uint64_t)d.n * 4 * rp * kp * 2;          // tables (either path, upper bound)
         bytes += 2 * us * 4 * b * (((uint64_t)d.m_src + 31) / 32 * 32);   // source digit planes
         bytes += 2 * ut2 * 4 * b * (((uint64_t)d.m_tgt + 31) / 32 * 32);  // mid digit planes
+        bytes += (us + ut2) * ((uint64_t)d.n + 1) * 4;                    // incidence tables
     } else {
         bytes += 6 * (rows + cols) * b * 4 + 64ull * (rows + cols);
         if (d.kind == PGW_KIND_CYCLIC_AUGMENTED || d.kind == PGW_KIND_CYCLIC_REPLACEMENT)

@@ -30,7 +30,8 @@ struct DenseSweep {
     uint64_t v_in = 0;        // input subsets
     uint32_t r_out = 0;       // output rows per subset panel
     uint32_t k_in = 0;        // input rows per subset panel
-    const uint64_t* binom = nullptr;  // device C(a,b) table, 64 x 64 (MFMA path)
+    const uint64_t* binom = nullptr;  // device C(a,b) table, 64 x 64
+    const uint32_t* inc = nullptr;    // MFMA path: incidence table (mfma_incidence_table)
 };
 
 // plain u64 path ------------------------------------------------------------
@@ -105,6 +106,14 @@ void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, ui
                       const uint32_t* src, bool transpose_src);
 // per-sweep digit-plane layout
 uint64_t limb_plane_bytes(uint64_t v, uint32_t k_in, uint32_t b, uint32_t nl);
+// Incidence table of one sweep direction for the MFMA kernels: n + 1 words
+// per output panel u (a deg_out-subset of [0, n), lexicographic rank order):
+// word 0 = mask of the tables incident to u (forward: i not in u; transpose:
+// i in u), word 1 + i = rank of the neighbour subset (u with i added or
+// removed) | sign << 31 (sign = parity of the elements of u below i), or ~0
+// when u is not incident to table i.
+std::vector<uint32_t> mfma_incidence_table(uint32_t n, uint32_t deg_out, bool fwd, uint64_t u_out,
+                                           uint64_t v_in);
 uint32_t limb_stride(uint32_t k_in);
 void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
                        uint32_t k_in, uint32_t b, uint32_t nl, const ModP& m, hipStream_t st);

@@ -54,6 +54,46 @@ void build_mfma_table(MfmaTable& t, uint32_t n, uint32_t rows, uint32_t k_in, ui
     t.zero = t.d + digits;
 }
 
+// Incidences of every output panel, computed once per operator so that a
+// sweep workgroup reads its panels' codes in one coalesced pass instead of
+// unranking and ranking subsets through chains of dependent lookups.
+// (cuda/cuprym_cyclic.cu:631-727 builds the same incidences and signs.)
+std::vector<uint32_t> mfma_incidence_table(uint32_t n, uint32_t deg_out, bool fwd, uint64_t u_out,
+                                           uint64_t v_in) {
+    require(n <= 32 && (fwd || deg_out >= 1), "incidence table: bad subset degree");
+    require(v_in < (1ULL << 31), "incidence table: neighbour ranks need 31 bits");
+    struct Pascal {
+        uint64_t c[33][33] = {};
+        uint64_t operator()(int a, int b) const { return (a < 0 || b < 0 || b > a) ? 0ULL : c[a][b]; }
+    } C;
+    for (int a = 0; a <= 32; ++a) {
+        C.c[a][0] = 1;
+        for (int b = 1; b <= a; ++b) C.c[a][b] = C.c[a - 1][b - 1] + (b < a ? C.c[a - 1][b] : 0);
+    }
+    const uint64_t w = (uint64_t)n + 1;
+    std::vector<uint32_t> t(u_out * w);
+    for (uint64_t u = 0; u < u_out; ++u) {
+        const uint64_t mask = unrank_mask(u, (int)n, (int)deg_out, C);
+        uint32_t* row = t.data() + u * w;
+        uint32_t on = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t bit = 1ULL << i;
+            uint32_t code = 0xffffffffu;
+            if (fwd ? !(mask & bit) : (mask & bit)) {
+                const uint32_t neg = (uint32_t)__builtin_popcountll(mask & (bit - 1)) & 1u;
+                const uint64_t nb = fwd ? rank_mask(mask | bit, (int)n, (int)deg_out + 1, C)
+                                        : rank_mask(mask & ~bit, (int)n, (int)deg_out - 1, C);
+                require(nb < v_in, "incidence table: neighbour rank out of range");
+                code = (uint32_t)nb | (neg << 31);
+                on |= 1u << i;
+            }
+            row[1 + i] = code;
+        }
+        row[0] = on;
+    }
+    return t;
+}
+
 // ---------------------------------------------------------------------------
 // digit planes of a block
 // ---------------------------------------------------------------------------

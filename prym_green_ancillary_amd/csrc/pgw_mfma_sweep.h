@@ -50,14 +50,6 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-// binomial coefficients read from a 64 x 64 device table (one per context)
-struct GlobalBinom {
-    const uint64_t* t;
-    __device__ uint64_t operator()(int a, int b) const {
-        return (a < 0 || b < 0 || b > a) ? 0ULL : t[a * 64 + b];
-    }
-};
-
 struct DigitSpec {
     int64_t maxpos;
     uint64_t bias;
@@ -78,7 +70,7 @@ struct ShiftWeights {
 };
 
 struct MfmaSweepArgs {
-    const uint64_t* binom;  // C(a,b) table, 64 x 64
+    const uint32_t* inc;    // incidence table of the sweep direction (n + 1 words per output panel)
     const int8_t* tabl;
     const int8_t* lp;
     const int8_t* zero;     // 1 KB of zero bytes (staging loads that have no data)
@@ -123,54 +115,30 @@ __device__ __forceinline__ bool mfma_block_tile(const MfmaSweepArgs& a, uint64_t
     return true;
 }
 
-// Binomial coefficients C(a, b), a <= 32, b < kBinW, staged in LDS: the
-// subset ranks of the prologue are chains of dependent lookups.
-constexpr int kBinW = 16;
-struct LdsBinom {
-    const uint64_t* t;
-    __device__ uint64_t operator()(int a, int b) const {
-        return (a < 0 || b < 0 || b > a) ? 0ULL : t[a * kBinW + b];
-    }
-};
-
-// Incidence codes of the workgroup's output panels: s_code[g*kMaxN + i] =
-// rank of the neighbour panel | sign << 31, or ~0 when panel g is not incident
-// to table i.  s_masks[0] = tables some panel meets (bit i), s_masks[1 + t] =
-// tables the panels of column tile t meet.  Ends with a barrier.
-template <bool FWD, int SLOTS, int SPT, int NT>
-__device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t cg, uint64_t* s_mask,
-                                                uint32_t* s_code, uint64_t* s_bin, uint32_t* s_masks) {
+// Incidence codes of the workgroup's output panels, read from the sweep's
+// incidence table (n + 1 words per output panel u, built once per operator:
+// word 0 the mask of tables incident to u, word 1 + i the code of table i):
+// s_code[g*kMaxN + i] = rank of the neighbour panel | sign << 31, or ~0 when
+// panel g is not incident to table i or lies past the last panel;
+// s_masks[0] = tables some panel meets (bit i), s_masks[1 + t] = tables the
+// panels of column tile t meet.  One coalesced pass and one barrier.
+template <int SLOTS, int SPT, int NT>
+__device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t cg, uint32_t* s_code,
+                                                uint32_t* s_masks) {
     constexpr int TILES_C = SLOTS / SPT;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < (kMaxN + 1) * kBinW; e += NT) {
-        const int x = e / kBinW, y = e % kBinW;
-        s_bin[e] = x <= (int)a.n ? a.binom[x * 64 + y] : 0ULL;
+    const int tid = threadIdx.x, n = (int)a.n;
+    const uint64_t w = (uint64_t)n + 1, u0 = cg * SLOTS;
+    for (int idx = tid; idx < SLOTS * n; idx += NT) {
+        const int g = idx / n, i = idx % n;
+        const uint64_t u = u0 + g;
+        s_code[g * kMaxN + i] = u < a.u_out ? a.inc[u * w + 1 + i] : 0xffffffffu;
     }
-    if (tid <= TILES_C) s_masks[tid] = 0;
-    __syncthreads();
-    LdsBinom C{s_bin};
-    if (tid < SLOTS) {
-        const uint64_t u = cg * SLOTS + tid;
-        s_mask[tid] = u < a.u_out ? unrank_mask(u, (int)a.n, (int)a.deg_out, C) : 0ULL;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < SLOTS * (int)a.n; idx += NT) {
-        const int g = idx / (int)a.n, i = idx % (int)a.n;
-        const uint64_t u = cg * SLOTS + g;
-        uint32_t code = 0xffffffffu;
-        if (u < a.u_out) {
-            const uint64_t mask = s_mask[g];
-            const bool bit = (mask >> i) & 1ULL;
-            if (FWD ? !bit : bit) {
-                const uint32_t neg = (uint32_t)__popcll(mask & ((1ULL << i) - 1)) & 1u;
-                const uint64_t nb = FWD ? rank_mask(mask | (1ULL << i), (int)a.n, (int)a.deg_out + 1, C)
-                                        : rank_mask(mask & ~(1ULL << i), (int)a.n, (int)a.deg_out - 1, C);
-                code = (uint32_t)nb | (neg << 31);
-                atomicOr(&s_masks[0], 1u << i);
-                atomicOr(&s_masks[1 + g / SPT], 1u << i);
-            }
-        }
-        s_code[g * kMaxN + i] = code;
+    if (tid <= TILES_C) {
+        const int g0 = tid == 0 ? 0 : (tid - 1) * SPT, g1 = tid == 0 ? SLOTS : tid * SPT;
+        uint32_t m = 0;
+        for (int g = g0; g < g1; ++g)
+            if (u0 + g < a.u_out) m |= a.inc[(u0 + g) * w];
+        s_masks[tid] = m;
     }
     __syncthreads();
 }
@@ -282,9 +250,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
 
     __shared__ __align__(16) int8_t s_a[2][A_BYTES];
     __shared__ __align__(16) int8_t s_b[2][B_BYTES];
-    __shared__ uint64_t s_mask[SLOTS];
     __shared__ uint32_t s_code[SLOTS * kMaxN];
-    __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
     __shared__ uint32_t s_masks[1 + TILES_C];
 
     uint64_t cg;
@@ -293,7 +259,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
     const int tid = threadIdx.x, wave = tid / 64;
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
-    mfma_incidences<FWD, SLOTS, SPT, NT>(a, cg, s_mask, s_code, s_bin, s_masks);
+    mfma_incidences<SLOTS, SPT, NT>(a, cg, s_code, s_masks);
 
     v4i acc[RT][CT][NS];
     static_for<RT>([&](auto rt) {
@@ -537,14 +503,14 @@ __device__ __forceinline__ void wait_vmcnt() {
 }
 
 // Waves per SIMD the direct-to-LDS kernel is compiled for: two workgroups
-// per CU when two copies of its LDS (stages, incidence codes, binomials) fit
+// per CU when two copies of its LDS (stages, incidence codes and masks) fit
 // in the 160 KB of a CU, so one workgroup's staging and fragment reads run
 // under the other's MFMAs; at least two waves per SIMD either way.
 template <int B, int NL, int RT, int CT, int WR, int WC, int STG>
 constexpr int dma_waves_per_simd() {
     constexpr int rows = WR * RT * 16, tiles_c = WC * CT, slots = tiles_c * (16 / B);
     constexpr int stage = NL * rows * kLdsStride + tiles_c * NL * 16 * kLdsStride;
-    constexpr int lds = STG * stage + slots * (8 + 4 * kMaxN) + (kMaxN + 1) * kBinW * 8 + 4 * (1 + tiles_c) + 64;
+    constexpr int lds = STG * stage + slots * 4 * kMaxN + 4 * (1 + tiles_c) + 64;
     constexpr int wgs = 2 * lds <= 160 * 1024 ? 2 : 1;
     constexpr int w = wgs * WR * WC / 4;
     return w < 2 ? 2 : w;
@@ -572,9 +538,7 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
     __shared__ __align__(1024) int8_t s_st0[A_BYTES + B_BYTES];
     __shared__ __align__(1024) int8_t s_st1[A_BYTES + B_BYTES];
     __shared__ __align__(1024) int8_t s_st2[STG == 3 ? A_BYTES + B_BYTES : 16];
-    __shared__ uint64_t s_mask[SLOTS];
     __shared__ uint32_t s_code[SLOTS * kMaxN];
-    __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
     __shared__ uint32_t s_masks[1 + TILES_C];
 
     uint64_t cg;
@@ -583,7 +547,7 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
-    mfma_incidences<FWD, SLOTS, SPT, NT>(a, cg, s_mask, s_code, s_bin, s_masks);
+    mfma_incidences<SLOTS, SPT, NT>(a, cg, s_code, s_masks);
 
     v4i acc[RT][CT][NS];
     static_for<RT>([&](auto rt) {
@@ -805,9 +769,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
     __shared__ __align__(1024) int8_t s_st0[B_BYTES];
     __shared__ __align__(1024) int8_t s_st1[B_BYTES];
     __shared__ __align__(1024) int8_t s_st2[STG == 3 ? B_BYTES : 16];
-    __shared__ uint64_t s_mask[SLOTS];
     __shared__ uint32_t s_code[SLOTS * kMaxN];
-    __shared__ uint64_t s_bin[(kMaxN + 1) * kBinW];
     __shared__ uint32_t s_masks[1 + TILES_C];
     __shared__ const int8_t* s_bptr[SLOTS * (kMaxN + 1)];  // digit plane of panel g's table-t neighbour
 
@@ -817,7 +779,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
-    mfma_incidences<FWD, SLOTS, SPT, NT>(a, cg, s_mask, s_code, s_bin, s_masks);
+    mfma_incidences<SLOTS, SPT, NT>(a, cg, s_code, s_masks);
     // s_bptr[g * (kMaxN + 1) + t]: base of the digit plane (sign and limb 0)
     // that panel g reads at table t, or the zero table when g is not incident
     // to t (t = n included: the missing partner of a tail pair).  The zero
@@ -840,7 +802,6 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
     });
 
     const uint32_t k_pad = a.k_pad;
-    const uint64_t plane = (uint64_t)NL * B * k_pad;
     const uint32_t limb_block_b = B * k_pad;
     const uint32_t limb_block_a = a.r_pad * k_pad;
     const uint32_t table_stride = NL * limb_block_a;
@@ -1044,10 +1005,10 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     constexpr int SLOTS = WC * CT * (16 / B);
     constexpr int ROWS = WR * RT * 16;
     require(s.n <= (uint32_t)kMaxN, "MFMA sweep supports n <= 32");
-    require(s.k < (uint32_t)kBinW, "MFMA sweep supports k < 16");
+    require(s.k < 16u, "MFMA sweep supports k < 16");
     MfmaSweepArgs a;
-    a.binom = s.binom;
-    require(a.binom != nullptr, "MFMA sweep needs the device binomial table");
+    a.inc = s.inc;
+    require(a.inc != nullptr, "MFMA sweep needs the operator's incidence table");
     a.tabl = t.d;
     a.lp = lp;
     a.zero = t.zero;

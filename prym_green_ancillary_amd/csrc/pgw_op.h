@@ -112,6 +112,7 @@ struct pgw_op {
     uint32_t nl = 0;
     int8_t* d_lp_src = nullptr;
     int8_t* d_lp_mid = nullptr;
+    uint32_t* d_inc[2] = {nullptr, nullptr};  // incidence tables: forward, transpose sweeps
     // cyclic eliminated
     PhiDevice dm, d2;
     uint32_t* d_zoff = nullptr;
