@@ -144,19 +144,14 @@ __device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t
 }
 
 // Epilogue: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile; fold
-// the digit shifts with w_s = 256^s mod p, reduce, scale, store.  The int32
-// shift sums are first made residues: |acc| < 2^31.
-//  * p = 2^31 - 1: acc + p (if negative) then one conditional subtract; w_s =
-//    2^(8s mod 31), so r * w_s mod p is a 31-bit rotation; the 7 terms sum
-//    below 2^34 and fold once (2^31 == 1).
-//  * 1024 <= p <= 65536 (two digits): acc - floor(acc / p) p from a float
-//    quotient (off by at most one), corrected; r_s * w_s < 2^32, three
-//    terms, one reduction.
-//  * otherwise: 64-bit Barrett per shift.
-__device__ __forceinline__ uint32_t rotl31(uint32_t r, int e) {
-    return e == 0 ? r : (((r << e) | (r >> (31 - e))) & 0x7fffffffu);
-}
-
+// the digit shifts to one residue, scale, store.  |acc_s| <= 2^31.
+//  * p = 2^31 - 1: 256^s == 2^(8s mod 31), so sum_s acc_s 2^(8s mod 31) is
+//    the value mod p and is exact in 64 bits: |sum| <= 2^31 (1 + 2 + 2^8 +
+//    2^9 + 2^16 + 2^17 + 2^24) < 2^56 - 2^25 = p 2^25; offset by p 2^25 it
+//    lies in (0, 2^57) and folds twice (2^31 == 1) to at most p + 1.
+//  * two digits (p <= 65536): sum_s acc_s 256^s is exact, |sum| < 2^48 <=
+//    p 2^47; offset by p 2^47 it is one 64-bit Barrett reduction.
+//  * otherwise: 64-bit Barrett per shift and weights w_s = 256^s mod p.
 template <int B, int NL, int RT, int CT, int WR, int WC>
 __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, uint32_t wg_row0,
                                            v4i (&acc)[RT][CT][2 * NL - 1]) {
@@ -169,32 +164,18 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
     const int my_slot = cc / B, my_c = cc % B;
     const uint32_t p = a.m.p;
     const bool mersenne = NL == 4 && p == 0x7fffffffu;
-    const float inv_p = 1.0f / (float)p;
-    const bool small_digits = NL == 2 && p >= 1024;  // float quotient exact to +-1
+    // (f & p) + (f >> 31) == f (mod 2^31 - 1)
+    auto fold31 = [](uint64_t f) { return (f & 0x7fffffffULL) + (f >> 31); };
     auto fold = [&](const int32_t (&x)[NS]) -> uint32_t {
         if (NL == 4 && mersenne) {
-            uint64_t sum = 0;
-            static_for<NS>([&](auto s) {
-                int64_t t = x[s];
-                uint32_t r = (uint32_t)(t < 0 ? t + (int64_t)p : t);
-                r = r >= p ? r - p : r;
-                sum += rotl31(r, (8 * decltype(s)::value) % 31);
-            });
-            uint32_t v = (uint32_t)((sum & 0x7fffffffu) + (sum >> 31));
+            uint64_t t = (uint64_t)p << 25;
+            static_for<NS>([&](auto s) { t += (uint64_t)(int64_t)x[s] << ((8 * decltype(s)::value) % 31); });
+            const uint32_t v = (uint32_t)fold31(fold31(t));
             return v >= p ? v - p : v;
-        } else if (NL == 2 && small_digits) {
-            uint64_t sum = 0;
-            static_for<NS>([&](auto s) {
-                const int32_t t = x[s];
-                const int32_t q = (int32_t)floorf((float)t * inv_p);
-                int64_t r = (int64_t)t - (int64_t)q * p;  // |error| < 2p from the float quotient
-                r = r < 0 ? r + p : r;
-                r = r < 0 ? r + p : r;
-                r = r >= p ? r - p : r;
-                r = r >= p ? r - p : r;
-                sum += (uint64_t)r * a.sw.w[s];
-            });
-            return mod_u64(sum, a.m);
+        } else if (NL == 2) {
+            uint64_t t = (uint64_t)p << 47;
+            static_for<NS>([&](auto s) { t += (uint64_t)(int64_t)x[s] << (8 * decltype(s)::value); });
+            return mod_u64(t, a.m);
         } else {
             const uint64_t off = (uint64_t)p << 32;  // multiple of p above 2^31
             uint64_t s_lo = 0, s_hi = 0;
@@ -216,7 +197,14 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
                 static_for<NS>([&](auto s) { x[s] = acc[rt][ct][s][decltype(v)::value]; });
                 uint32_t val = fold(x);
                 const uint64_t orow = u * a.r_out + row;
-                if (a.scale) val = mulmod(val, a.scale[orow], a.m);
+                if (a.scale) {
+                    if (mersenne) {  // product < 2^62: two folds leave at most p + 1
+                        const uint32_t q = (uint32_t)fold31(fold31((uint64_t)val * a.scale[orow]));
+                        val = q >= p ? q - p : q;
+                    } else {
+                        val = mulmod(val, a.scale[orow], a.m);
+                    }
+                }
                 a.out[orow * B + my_c] = val;
             });
         });

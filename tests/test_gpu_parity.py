@@ -128,9 +128,10 @@ def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12):
 
 # ----------------------------------------------------------------- synthetic --
 
-# primes chosen for each digit fold of the MFMA epilogue: 2^31-1 (Mersenne
-# rotations), 2147483629 (general 4-digit Barrett), 65521 and 1031 (2-digit
-# float quotient), 1021 and 3 (2-digit Barrett, p < 1024)
+# primes chosen for each digit fold of the MFMA epilogue: 2^31-1 (exact
+# shifted sum, Mersenne folds), 2147483629 (general 4-digit Barrett), and for
+# the exact 2-digit sum offset by p 2^47: 65521 (largest), 1031 and 1021, and 3
+# (the smallest offset above the sum's bound)
 @pytest.mark.parametrize("p,b", [(65521, 8), (P31, 16), (P31, 4), (65521, 4), (P31, 8), (65521, 16),
                                  (2147483629, 16), (1031, 8), (1021, 16), (3, 4)])
 def test_small_synthetic_plain_and_mfma_vs_oracle(torch_cuda, ctx, p, b):
