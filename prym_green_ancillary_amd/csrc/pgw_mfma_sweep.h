@@ -187,6 +187,18 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
             return addmod(mod_u64(s_lo, a.m), mod_u64(s_hi, a.m), p);
         }
     };
+    // the lane's scale factors, all loaded before the first store (a load
+    // after a store to `out` would otherwise wait for it: the pointers may alias)
+    uint32_t sc[RT][CT][4];
+    static_for<RT>([&](auto rt) {
+        static_for<CT>([&](auto ct) {
+            const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
+            static_for<4>([&](auto v) {
+                const uint32_t row = wg_row0 + (wr * RT + rt) * 16 + (lane / 16) * 4 + v;
+                sc[rt][ct][v] = a.scale && u < a.u_out && row < a.r_out ? a.scale[u * a.r_out + row] : 1u;
+            });
+        });
+    });
     static_for<RT>([&](auto rt) {
         static_for<CT>([&](auto ct) {
             const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
@@ -198,11 +210,12 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
                 uint32_t val = fold(x);
                 const uint64_t orow = u * a.r_out + row;
                 if (a.scale) {
+                    const uint32_t f = sc[rt][ct][v];
                     if (mersenne) {  // product < 2^62: two folds leave at most p + 1
-                        const uint32_t q = (uint32_t)fold31(fold31((uint64_t)val * a.scale[orow]));
+                        const uint32_t q = (uint32_t)fold31(fold31((uint64_t)val * f));
                         val = q >= p ? q - p : q;
                     } else {
-                        val = mulmod(val, a.scale[orow], a.m);
+                        val = mulmod(val, f, a.m);
                     }
                 }
                 a.out[orow * B + my_c] = val;
