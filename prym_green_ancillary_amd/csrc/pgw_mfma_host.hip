@@ -124,7 +124,16 @@ __global__ __launch_bounds__(256) void k_limb_split(const uint32_t* __restrict__
         if (k < k_in) {
             const uint64_t row = v * k_in + k;
             x = in[row * B + c];
-            if (scale) x = mulmod(x, scale[row], m);
+            if (scale) {
+                if (m.p == 0x7fffffffu) {  // x * scale < 2^62: two Mersenne folds leave at most p + 1
+                    uint64_t t = (uint64_t)x * scale[row];
+                    t = (t & 0x7fffffffULL) + (t >> 31);
+                    t = (t & 0x7fffffffULL) + (t >> 31);
+                    x = (uint32_t)(t >= m.p ? t - m.p : t);
+                } else {
+                    x = mulmod(x, scale[row], m);
+                }
+            }
         }
         const uint32_t xn = x ? m.p - x : 0u;
         const uint64_t u0 = digits_biased(x, m.p, ds), u1 = digits_biased(xn, m.p, ds);
