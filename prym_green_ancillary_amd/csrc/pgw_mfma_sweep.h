@@ -152,9 +152,32 @@ __device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t
 //  * two digits (p <= 65536): sum_s acc_s 256^s is exact, |sum| < 2^48 <=
 //    p 2^47; offset by p 2^47 it is one 64-bit Barrett reduction.
 //  * otherwise: 64-bit Barrett per shift and weights w_s = 256^s mod p.
+// The lane's scale factors (1 without a scale vector), all loaded before the
+// epilogue's first store: a load after a store to `out` would wait for it
+// (the pointers may alias).  Loading them before the K loop instead costs 16
+// VGPRs there, which makes the register-A kernels spill.
+template <int B, int RT, int CT, int WR, int WC>
+__device__ __forceinline__ void mfma_scales(const MfmaSweepArgs& a, uint64_t cg, uint32_t wg_row0,
+                                            uint32_t (&sc)[RT][CT][4]) {
+    constexpr int SPT = 16 / B;
+    constexpr int SLOTS = WC * CT * SPT;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int wr = wave / WC, wc = wave % WC;
+    const int my_slot = (lane % 16) / B;
+    static_for<RT>([&](auto rt) {
+        static_for<CT>([&](auto ct) {
+            const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
+            static_for<4>([&](auto v) {
+                const uint32_t row = wg_row0 + (wr * RT + rt) * 16 + (lane / 16) * 4 + v;
+                sc[rt][ct][v] = a.scale && u < a.u_out && row < a.r_out ? a.scale[u * a.r_out + row] : 1u;
+            });
+        });
+    });
+}
+
 template <int B, int NL, int RT, int CT, int WR, int WC>
 __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, uint32_t wg_row0,
-                                           v4i (&acc)[RT][CT][2 * NL - 1]) {
+                                           v4i (&acc)[RT][CT][2 * NL - 1], const uint32_t (&sc)[RT][CT][4]) {
     constexpr int NS = 2 * NL - 1;
     constexpr int SPT = 16 / B;
     constexpr int SLOTS = WC * CT * SPT;
@@ -187,18 +210,6 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
             return addmod(mod_u64(s_lo, a.m), mod_u64(s_hi, a.m), p);
         }
     };
-    // the lane's scale factors, all loaded before the first store (a load
-    // after a store to `out` would otherwise wait for it: the pointers may alias)
-    uint32_t sc[RT][CT][4];
-    static_for<RT>([&](auto rt) {
-        static_for<CT>([&](auto ct) {
-            const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
-            static_for<4>([&](auto v) {
-                const uint32_t row = wg_row0 + (wr * RT + rt) * 16 + (lane / 16) * 4 + v;
-                sc[rt][ct][v] = a.scale && u < a.u_out && row < a.r_out ? a.scale[u * a.r_out + row] : 1u;
-            });
-        });
-    });
     static_for<RT>([&](auto rt) {
         static_for<CT>([&](auto ct) {
             const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
@@ -456,7 +467,9 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
         i2 = ni2;
         k0 = nk;
     }
-    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
+    uint32_t sc[RT][CT][4];
+    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, sc);
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -734,7 +747,9 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
             }
         }
     }
-    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
+    uint32_t sc[RT][CT][4];
+    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, sc);
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -994,7 +1009,9 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
             }
         }
     }
-    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc);
+    uint32_t sc[RT][CT][4];
+    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, sc);
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc, sc);
 }
 
 // kernel variants of a tile configuration: DMA 0 register-staged, 2 or 3
