@@ -265,8 +265,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", choices=["sector", "large", "batch", "serve"], default="sector",
-                    help="sector: one mid-size sector (default); large: the N=53.8M sector; "
+    ap.add_argument("--workload", choices=["sector", "large", "batch", "serve"], default="large",
+                    help="large: the N=53.8M sector (default); sector: the mid-size sector; "
                          "batch: 64 uneven sectors; serve: Poisson job stream")
     ap.add_argument("--rounds", type=int, default=4, help="batch: sequence rounds per sector")
     ap.add_argument("--sectors", type=int, default=64, help="batch: sectors in the mix")
@@ -347,7 +347,7 @@ def main():
     ops_per_sweep = 2.0 * macs_per_sweep(n, k, M, Mp, b) * nl * nl  # int8 ops, algorithmic
     achieved = ops_per_sweep / (sweep_ms * 1e-3) / 1e12
     peak = 5000.0  # TOP/s dense int8, MI355X
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None,
                 "kernel": "k_dense_mfma", "sweep_ms": round(sweep_ms, 4),
                 "sweeps_per_apply": sweeps, "tile_configs": tile_choice,

@@ -8,6 +8,7 @@ import json
 import subprocess
 import sys
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -84,14 +85,14 @@ def test_interrupt_while_waiting_terminates_and_reaps_the_child(tmp_path, monkey
         run.command("after", [sys.executable, "-c", "pass"])
 
 
-def test_resources_serve_priority_zero_first():
-    res = pl.Resources(["gpu0"])
+def test_lease_pool_serves_priority_zero_first():
+    pool = pl.LeasePool(["gpu0"])
     order, started = [], []
     hold = threading.Event()
 
     def user(tag, prio):
         started.append(tag)
-        with res.reserve(prio):
+        with pool.lease(1, prio):
             order.append(tag)
             if tag == "first":
                 hold.wait(5)
@@ -104,14 +105,44 @@ def test_resources_serve_priority_zero_first():
     late_low.start()
     while "low" not in started:
         pass
+    time.sleep(0.05)
     high = threading.Thread(target=user, args=("high", 0))
     high.start()
     while "high" not in started:
         pass
+    time.sleep(0.05)
     hold.set()
     for t in (t0, late_low, high):
         t.join(5)
     assert order == ["first", "high", "low"]
+    assert pool.available == ["gpu0"]
+
+
+def test_lease_pool_large_request_is_not_overtaken_and_cancel_leaves_the_queue():
+    pool = pl.LeasePool(range(4))
+    held = pool.acquire(3)
+    got = []
+    big = threading.Thread(target=lambda: got.append(pool.acquire(2)))
+    big.start()
+    time.sleep(0.05)
+    # a later one-unit request of the same priority waits behind the two-unit one
+    small = threading.Thread(target=lambda: got.append(pool.acquire(1)))
+    small.start()
+    time.sleep(0.05)
+    assert got == []
+    pool.release(held)
+    big.join(5)
+    small.join(5)
+    assert sorted(len(g) for g in got) == [1, 2]
+    # an abandoned wait (cancelled campaign) leaves no request behind
+    cancelled = threading.Event()
+    cancelled.set()
+    with pytest.raises(RuntimeError, match="cancelled"):
+        pool.acquire(4, 1, cancelled)
+    for g in got:
+        pool.release(g)
+    assert sorted(pool.available) == [0, 1, 2, 3]
+    assert sorted(pool.acquire(4)) == [0, 1, 2, 3]
 
 
 def _fake_jobs(run, n):
