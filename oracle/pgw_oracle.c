@@ -916,12 +916,17 @@ void orc_apply_square(orc_op* op, const uint32_t* in, uint32_t* out, size_t b) {
 void orc_sweep_partial(orc_op* op, int which, const uint32_t* in, uint32_t* out, size_t b,
                        size_t row0, size_t row1) {
     if (op->kind != ORC_SYNTHETIC) return;
-    if (which == 0) phi_forward(&op->d_op, in, out, b, op->p, row0, row1);
-    else phi_transpose_range(&op->e_op, in, out, b, op->p, row0, row1);
+    switch (which) {
+        case 0: phi_forward(&op->d_op, in, out, b, op->p, row0, row1); break;          /* D   */
+        case 1: phi_transpose_range(&op->e_op, in, out, b, op->p, row0, row1); break;  /* E   */
+        case 2: phi_forward(&op->e_op, in, out, b, op->p, row0, row1); break;          /* E^T */
+        case 3: phi_transpose_range(&op->d_op, in, out, b, op->p, row0, row1); break;  /* D^T */
+        default: break;
+    }
 }
 
 size_t orc_sweep_rows(const orc_op* op, int which) {
-    return which == 0 ? op->d_op.rows : op->e_op.cols;
+    return (which == 0 || which == 2) ? op->d_op.rows : op->e_op.cols;
 }
 
 /* packed upper triangle: ut(i,j) = i(2b-i+1)/2 + (j-i), cuda/prym_cuda_helpers.h:49-55 */

@@ -111,12 +111,14 @@ void orc_apply_backward(orc_op* op, const uint32_t* in, uint32_t* out, size_t b)
 /* S*X = Dc F^T P^T Dr P F Dc X (cuprym_cyclic.cu:2408-2418, 1939-1966). */
 void orc_apply_square(orc_op* op, const uint32_t* in, uint32_t* out, size_t b);
 /* One Koszul sweep of a synthetic operator restricted to output rows
- * [row0,row1): which = 0 forward gather with table T (the D sweep),
- * 1 transpose gather with table E (the E sweep).  Used only to time a bounded
- * CPU sample of the apply; `out` must hold the whole sweep output. */
+ * [row0,row1): which = 0 the D sweep (forward gather, table T, N -> R),
+ * 1 the E sweep (transpose gather, table E, R -> N), 2 the E^T sweep
+ * (forward gather, table E, N -> R), 3 the D^T sweep (transpose gather,
+ * table T, R -> N).  Used to time a bounded CPU sample of the apply; `out`
+ * must hold the whole sweep output. */
 void orc_sweep_partial(orc_op* op, int which, const uint32_t* in, uint32_t* out, size_t b,
                        size_t row0, size_t row1);
-/* Output rows of the two synthetic sweeps (R for the D sweep, N for E). */
+/* Output rows of a synthetic sweep (R for D and E^T, N for E and D^T). */
 size_t orc_sweep_rows(const orc_op* op, int which);
 
 /* upper-triangle Gram, packed seq[ut(i,j)] (cuda/prym_cuda_helpers.h:87-137,318-354) */

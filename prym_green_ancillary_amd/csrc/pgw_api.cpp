@@ -14,33 +14,15 @@ using namespace pgw;
 
 namespace {
 
-PhiDevice upload_phi(DeviceArena& A, const PhiHost& h) {
-    PhiDevice d;
-    d.nI = h.nI; d.nB = h.nB; d.nA = h.nA; d.r = h.r;
-    d.rows = h.row_rank.size();
-    d.cols = h.col_rank.size();
-    d.row_subsets = h.row_subsets;
-    d.col_subsets = h.col_subsets;
-    d.row_width = h.row_width;
-    d.col_width = h.col_width;
-    d.mu = A.upload(h.mu);
-    d.wV = A.upload(h.wV);
-    d.wA0 = A.upload(h.wA0);
-    d.wA1 = A.upload(h.wA1);
-    d.a0_off = A.upload(h.a0_off);
-    d.a0_idx = A.upload(h.a0_idx);
-    d.a1_off = A.upload(h.a1_off);
-    d.a1_idx = A.upload(h.a1_idx);
-    d.row_off = A.upload(h.row_off);
-    d.row_rank = A.upload(h.row_rank);
-    d.row_beta = A.upload(h.row_beta);
-    d.col_off = A.upload(h.col_off);
-    d.col_rank = A.upload(h.col_rank);
-    d.col_a = A.upload(h.col_a);
-    d.row_inc = A.upload(h.row_inc_code);
-    d.row_inc_rank = A.upload(h.row_inc_rank);
-    d.col_inc = A.upload(h.col_inc_code);
-    d.col_inc_rank = A.upload(h.col_inc_rank);
+// A sparse map in its device (sliced-ELL) form, owned by the operator's arena.
+SellDevice upload_sell(DeviceArena& A, const CsrHost& csr) {
+    const SellHost h = csr_to_sell(csr);
+    SellDevice d;
+    d.rows = h.rows;
+    d.cols = h.cols;
+    d.nnz = h.nnz;
+    d.slice = A.upload(h.slice);
+    d.ent = A.upload(h.ent);
     return d;
 }
 
@@ -121,59 +103,23 @@ void forward_into_target(pgw_op* op, const uint32_t* x, hipStream_t st) {
             run_sweep(op, 1, op->d_mid, nullptr, op->d_target, nullptr, st);
         }
     } else {
-        // Dc-scaled input: the base rows then, for a deformation variant, the
-        // dense-column rows (replacement: kept columns carried to their base
-        // positions, cuda/cuprym_deformation.cu:2787-2821)
-        if (op->kind == PGW_KIND_CYCLIC_REPLACEMENT)
-            launch_complement_scale(x, op->d_dc, op->d_scaled, op->base_cols, op->drop0, op->drop1,
-                                    op->dense_cols, op->b, m, st);
-        else
-            launch_scale_rows(x, op->d_dc, op->d_scaled, op->cols, op->b, m, st);
-        launch_cyclic_fwd(op->dm, op->d_scaled, op->d_t, nullptr, op->b, m, st);
-        launch_elim_y(op->dm, op->d2, op->d_zoff, op->d_zkappa, op->d_R, op->d_Bk, op->n_v, op->kdim,
-                      op->d_scaled + op->x_cols * op->b, op->d_t, op->d_y, op->b, m, st);
-        launch_cyclic_fwd(op->d2, op->d_y, op->d_target, nullptr, op->b, m, st);
-        // [F | -dense] (augmented) or [F' | dense] (replacement)
-        if (op->dense_cols)
-            launch_dense_cols_axpy(op->d_target, op->d_dense, op->d_scaled + op->base_cols * op->b,
-                                   op->rows, op->dense_cols, op->b,
-                                   op->kind == PGW_KIND_CYCLIC_AUGMENTED, m, st);
+        // F (Dc x) of the cyclic operator: the first two sparse maps
+        launch_sell_spmm(op->m1, x, op->d_y, op->b, m, st);
+        launch_sell_spmm(op->m2, op->d_y, op->d_target, op->b, m, st);
     }
 }
 
-// out = Dc F^T (op->d_target)
+// out = Dc F^T (op->d_target), synthetic operator
 void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
     const ModP& m = op->m;
-    if (op->kind == PGW_KIND_SYNTHETIC) {
-        if (op->method == PGW_METHOD_MFMA) {
-            launch_limb_split(op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, op->b, op->nl, m, st);
-            run_sweep(op, 2, nullptr, op->d_lp_src, op->d_mid, nullptr, st);
-            launch_limb_split(op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, op->b, op->nl, m, st);
-            run_sweep(op, 3, nullptr, op->d_lp_mid, out, op->d_dc, st);
-        } else {
-            run_sweep(op, 2, op->d_target, nullptr, op->d_mid, nullptr, st);
-            run_sweep(op, 3, op->d_mid, nullptr, out, op->d_dc, st);
-        }
+    if (op->method == PGW_METHOD_MFMA) {
+        launch_limb_split(op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, op->b, op->nl, m, st);
+        run_sweep(op, 2, nullptr, op->d_lp_src, op->d_mid, nullptr, st);
+        launch_limb_split(op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, op->b, op->nl, m, st);
+        run_sweep(op, 3, nullptr, op->d_lp_mid, out, op->d_dc, st);
     } else {
-        // the base adjoint lands in `out` (plain and augmented) or, for the
-        // replacement, in a base-sized block whose kept rows are gathered
-        uint32_t* base_out = op->kind == PGW_KIND_CYCLIC_REPLACEMENT ? op->d_base_adj : out;
-        launch_cyclic_trn(op->d2, op->d_target, op->d_h, nullptr, op->b, m, st);
-        launch_elim_rt_h(op->dm, op->d2, op->d_R, op->n_v, op->d_h, op->d_tadj, op->b, m, st);
-        // -x_adj scaled by Dc: fold the sign into the scale (p - Dc)
-        launch_cyclic_trn(op->dm, op->d_tadj, base_out, op->d_base_negdc, op->b, m, st);
-        launch_elim_bt_h(op->d2, op->z_count, op->d_zrank, op->d_zkappa, op->d_Bk, op->kdim, op->d_h,
-                         base_out + op->x_cols * op->b, op->d_base_dc + op->x_cols, op->b, m, st);
-        if (op->dense_cols) {
-            const bool aug = op->kind == PGW_KIND_CYCLIC_AUGMENTED;
-            const uint64_t lead = aug ? op->base_cols : op->base_cols - 2;  // rows before the dense ones
-            if (!aug)
-                launch_complement_gather(op->d_base_adj, out, op->base_cols, op->drop0, op->drop1, op->b, st);
-            // dense^T (target), negated for [F | -dense], scaled by Dc
-            // (cuda/cuprym_deformation.cu:2822-2855)
-            launch_dense_cols_transpose(op->d_dense, op->d_target, op->rows, op->dense_cols, op->b,
-                                        op->d_dc, lead, out, aug, op->d_dense_scratch, m, st);
-        }
+        run_sweep(op, 2, op->d_target, nullptr, op->d_mid, nullptr, st);
+        run_sweep(op, 3, op->d_mid, nullptr, out, op->d_dc, st);
     }
 }
 
@@ -181,6 +127,16 @@ void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
 }  // namespace
 
 void pgw::apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st) {
+    if (op->kind != PGW_KIND_SYNTHETIC) {
+        // S = M1^T M2^T M3 M2 M1: Dc, the elimination and P^T Dr P are folded in
+        const ModP& m = op->m;
+        launch_sell_spmm(op->m1, in, op->d_y, op->b, m, st);
+        launch_sell_spmm(op->m2, op->d_y, op->d_target, op->b, m, st);
+        launch_sell_spmm(op->m3, op->d_target, op->d_t2, op->b, m, st);
+        launch_sell_spmm(op->m2t, op->d_t2, op->d_y, op->b, m, st);
+        launch_sell_spmm(op->m1t, op->d_y, out, op->b, m, st);
+        return;
+    }
     forward_into_target(op, in, st);
     launch_target_precond(op->d_target, op->d_pairs, op->pair_count, op->rounds, op->d_dr,
                           op->unpaired, op->rows, op->b, op->m, st);
@@ -346,6 +302,8 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
         op->m = make_modp(op->p);
         DeviceArena& A = op->arena;
         const uint32_t b = op->b;
+        CyclicLayout layout;
+        bool cyclic = false;
         if (desc->kind == PGW_KIND_SYNTHETIC) {
             require(desc->table_d && desc->table_e, "synthetic operator needs both tables");
             synthetic_dims(desc, op->rows, op->cols, op->mid);
@@ -418,44 +376,21 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             require(desc->method != PGW_METHOD_MFMA,
                     "the cyclic operator's character blocks run on the plain path", PGW_ERR_UNSUPPORTED);
             op->method = PGW_METHOD_PLAIN;
-            CyclicLayout L;
-            build_cyclic_layout(L, *desc);
-            op->rows = L.rows;
-            op->cols = L.cols;
-            op->base_cols = L.cols;
+            build_cyclic_layout(layout, *desc);
+            op->rows = layout.rows;
+            op->cols = layout.cols;
             if (is_deformation(desc->kind)) {
-                op->cols = deformation_cols(desc, L.cols, op->drop0, op->drop1);
-                op->dense_cols = desc->dense_cols;
-                const uint64_t cnt = op->rows * op->dense_cols;
+                uint32_t d0 = 0, d1 = 0;
+                op->cols = deformation_cols(desc, layout.cols, d0, d1);
+                const uint64_t cnt = op->rows * desc->dense_cols;
                 uint32_t dlo = 0, dhi = 0;
                 min_max_u32(desc->dense_block, cnt, dlo, dhi);
                 require(cnt == 0 || dhi < op->p, "dense block entries must be reduced modulo p");
-                op->d_dense = A.upload(desc->dense_block, cnt);
-                op->d_dense_scratch = A.alloc<uint64_t>(dense_cols_scratch_bytes(op->dense_cols, b) / 8);
-                if (desc->kind == PGW_KIND_CYCLIC_REPLACEMENT)
-                    op->d_base_adj = A.alloc<uint32_t>(op->base_cols * b);
             }
-            op->mid = L.d2.col_rank.size();
-            op->dm = upload_phi(A, L.dm);
-            op->d2 = upload_phi(A, L.d2);
-            op->d_zoff = A.upload(L.z_off);
-            op->d_zrank = A.upload(L.z_rank);
-            op->d_zkappa = A.upload(L.z_kappa);
             const uint32_t g = desc->genus, nv = g - 3;
-            op->n_v = nv;
-            op->kdim = desc->kernel_dim;
             for (uint64_t q = 0; q < (uint64_t)nv * nv * g; ++q)
                 require(desc->mu[q] < op->p, "mu.values entry is not reduced modulo the instance modulus");
-            op->d_R = A.upload(desc->right_inverse, (uint64_t)g * nv);
-            op->d_Bk = A.upload(desc->kernel_basis, (uint64_t)g * desc->kernel_dim);
-            op->x_cols = L.x_cols;
-            op->z_count = L.z_count;
-            // scaled input: the base rows, then the dense-column rows of a variant
-            op->d_scaled = A.alloc<uint32_t>((op->base_cols + op->dense_cols) * b);
-            op->d_t = A.alloc<uint32_t>(L.dm.row_rank.size() * b);
-            op->d_y = A.alloc<uint32_t>(op->mid * b);
-            op->d_h = A.alloc<uint32_t>(op->mid * b);
-            op->d_tadj = A.alloc<uint32_t>(L.dm.row_rank.size() * b);
+            cyclic = true;
         } else {
             throw Error(PGW_ERR_INVALID, "unknown operator kind");
         }
@@ -470,21 +405,6 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
         std::vector<uint32_t> negdc(op->cols);
         for (uint64_t q = 0; q < op->cols; ++q) negdc[q] = op->p - h_col_precond[q];
         op->d_negdc = A.upload(negdc);
-        // the base operator's source preconditioner: Dc itself, or for the
-        // replacement Dc carried to the base positions of the kept columns
-        // (0 at the two dropped columns, whose adjoint rows are discarded)
-        op->d_base_dc = op->d_dc;
-        op->d_base_negdc = op->d_negdc;
-        if (op->kind == PGW_KIND_CYCLIC_REPLACEMENT) {
-            std::vector<uint32_t> bdc(op->base_cols, 0), bneg(op->base_cols, 0);
-            for (uint64_t f = 0, j = 0; f < op->base_cols; ++f) {
-                if (f == op->drop0 || f == op->drop1) continue;
-                bdc[f] = h_col_precond[j++];
-                bneg[f] = op->p - bdc[f];
-            }
-            op->d_base_dc = A.upload(bdc);
-            op->d_base_negdc = A.upload(bneg);
-        }
         op->rounds = rowmix_rounds;
         op->pair_count = op->rows / 2;
         if (rowmix_rounds > 0 && op->pair_count > 0) {
@@ -510,6 +430,23 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             op->d_pairs = A.upload(h_pairs, total);
         } else {
             op->rounds = 0;
+        }
+        if (cyclic) {
+            CyclicMaps maps;
+            build_cyclic_maps(maps, layout, *desc, op->cols, h_col_precond, h_row_precond, op->rounds,
+                              h_pairs);
+            layout = CyclicLayout();
+            op->y_rows = maps.m1.rows;
+            op->mid = op->y_rows;
+            op->m1 = upload_sell(A, maps.m1);
+            op->m1t = upload_sell(A, csr_transpose(maps.m1));
+            maps.m1 = CsrHost();
+            op->m2 = upload_sell(A, maps.m2);
+            op->m2t = upload_sell(A, csr_transpose(maps.m2));
+            maps.m2 = CsrHost();
+            op->m3 = upload_sell(A, maps.m3);
+            op->d_y = A.alloc<uint32_t>(op->y_rows * b);
+            op->d_t2 = A.alloc<uint32_t>(op->rows * b);
         }
         op->d_target = A.alloc<uint32_t>(op->rows * b);
         op->d_gram = A.alloc<uint64_t>(gram_scratch_bytes(b) / sizeof(uint64_t));
@@ -586,6 +523,9 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
     uint32_t* d_B = nullptr;
     uint32_t* d_D = nullptr;
     uint32_t* d_seq = nullptr;
+    uint32_t* d_pos = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
     int rc = guarded([&] {
         require(op && h_v && h_seq && h_curv, "null operator or buffer");
         require(max_len >= 1, "sequence length must be positive");
@@ -599,23 +539,43 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
             min_max_u32(h_v, N * b, lo, hi);
             require(N * b == 0 || hi < op->p, "V entries must be reduced modulo p");
         }
-        PGW_HIP(hipMalloc(&d_B, N * b * sizeof(uint32_t)));
-        PGW_HIP(hipMalloc(&d_D, N * b * sizeof(uint32_t)));
-        PGW_HIP(hipMalloc(&d_seq, L * ut * sizeof(uint32_t)));
+        {
+            std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
+            PGW_HIP(hipMalloc(&d_B, N * b * sizeof(uint32_t)));
+            PGW_HIP(hipMalloc(&d_D, N * b * sizeof(uint32_t)));
+            PGW_HIP(hipMalloc(&d_seq, L * ut * sizeof(uint32_t)));
+            PGW_HIP(hipMalloc(&d_pos, sizeof(uint32_t)));
+            const uint32_t one = 1;  // the next sequence position, read on the device
+            PGW_HIP(hipMemcpy(d_pos, &one, sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
         PGW_HIP(hipMemcpyAsync(d_B, h_v, N * b * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        uint64_t pos = 0;
-        launch_gram(d_B, d_B, N, op->b, 0, d_seq + ut * pos, nullptr, op->d_gram, op->m, st);
-        ++pos;
+        launch_gram(d_B, d_B, N, op->b, 0, d_seq, nullptr, op->d_gram, op->m, st);  // V^T V
+        PGW_HIP(hipStreamSynchronize(st));
+        if (rounds) {
+            // one round (cuprym_cyclic.cu:2439-2449) captured once and replayed:
+            // D = S B; B^T D, D^T D; B = S D; B^T D, B^T B; pos += 4
+            std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
+            PGW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                apply_square_impl(op, d_B, d_D, st);
+                launch_gram(d_B, d_D, N, op->b, 1, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 0);
+                apply_square_impl(op, d_D, d_B, st);
+                launch_gram(d_B, d_D, N, op->b, 2, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 2);
+                launch_advance_pos(d_pos, 4, st);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            PGW_HIP(hipStreamEndCapture(st, &graph));
+            PGW_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
         for (uint64_t r = 0; r < rounds; ++r) {
-            apply_square_impl(op, d_B, d_D, st);
-            launch_gram(d_B, d_D, N, op->b, 1, d_seq + ut * pos, d_seq + ut * (pos + 1), op->d_gram, op->m, st);
-            pos += 2;
-            apply_square_impl(op, d_D, d_B, st);
-            launch_gram(d_B, d_D, N, op->b, 2, d_seq + ut * pos, d_seq + ut * (pos + 1), op->d_gram, op->m, st);
-            pos += 2;
-            if (progress && ((r + 1) % 64 == 0 || r + 1 == rounds)) {
+            PGW_HIP(hipGraphLaunch(exec, st));
+            if ((r + 1) % 64 == 0 || r + 1 == rounds) {
                 PGW_HIP(hipStreamSynchronize(st));  // also bounds the launch queue
-                progress(pos, L, user);
+                if (progress) progress(1 + 4 * (r + 1), L, user);
             }
         }
         PGW_HIP(hipMemcpyAsync(h_seq, d_seq, L * ut * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -623,9 +583,15 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
         PGW_HIP(hipStreamSynchronize(st));
         if (out_len) *out_len = L;
     });
-    if (d_B) (void)hipFree(d_B);
-    if (d_D) (void)hipFree(d_D);
-    if (d_seq) (void)hipFree(d_seq);
+    // a failed call may leave work queued that still reads these buffers
+    if (op) (void)hipStreamSynchronize(op->ctx->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (op) {
+        std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
+        for (uint32_t* p : {d_B, d_D, d_seq, d_pos})
+            if (p) (void)hipFree(p);
+    }
     return rc;
 }
 

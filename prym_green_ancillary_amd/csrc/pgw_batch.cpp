@@ -473,10 +473,10 @@ uint64_t estimate_job_bytes(const pgw_operator_desc& d, uint64_t rows, uint64_t 
         bytes += 2 * ut2 * 4 * b * (((uint64_t)d.m_tgt + 31) / 32 * 32);  // mid digit planes
         bytes += (us + ut2) * ((uint64_t)d.n + 1) * 4;                    // incidence tables
     } else {
-        bytes += 6 * (rows + cols) * b * 4 + 64ull * (rows + cols);
-        if (d.kind == PGW_KIND_CYCLIC_AUGMENTED || d.kind == PGW_KIND_CYCLIC_REPLACEMENT)
-            bytes += rows * d.dense_cols * 4 + 2 * (cols + d.dense_cols) * b * 4 +  // dense, base blocks
-                     dense_cols_scratch_bytes(d.dense_cols, (uint32_t)b) + 3 * (cols + 2) * 4;
+        // y', the mixed target and the five sparse maps of the chain (sliced
+        // ELL, 8 bytes per entry; rows of the paper instances hold < 100
+        // entries per map, padding included)
+        bytes += 3 * (rows + cols + 2) * b * 4 + 5 * 100 * 8 * (rows + cols + 64);
     }
     return bytes + (16ull << 20);
 }

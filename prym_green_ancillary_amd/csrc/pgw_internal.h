@@ -176,32 +176,6 @@ struct pgw_ctx {
 
 namespace pgw {
 
-// Character-restricted differential of the cyclic operator, as device arrays
-// (the reference's CudaCyclicPhiOperator, cuda/cyclic_cuda_helpers.h:268-296).
-struct PhiDevice {
-    uint32_t nI = 0, nB = 0, nA = 0, r = 1;
-    uint64_t rows = 0, cols = 0, row_subsets = 0, col_subsets = 0;
-    uint32_t row_width = 0, col_width = 0;
-    uint32_t* mu = nullptr;          // [nI][nB][nA]
-    uint16_t* wV = nullptr;
-    uint16_t* wA0 = nullptr;
-    uint16_t* wA1 = nullptr;
-    uint32_t* a0_off = nullptr;
-    uint16_t* a0_idx = nullptr;
-    uint32_t* a1_off = nullptr;
-    uint16_t* a1_idx = nullptr;
-    uint32_t* row_off = nullptr;     // [row_subsets+1]
-    uint32_t* row_rank = nullptr;    // [rows]
-    uint16_t* row_beta = nullptr;    // [rows]
-    uint32_t* col_off = nullptr;     // [col_subsets+1]
-    uint32_t* col_rank = nullptr;    // [cols]
-    uint16_t* col_a = nullptr;       // [cols]
-    uint32_t* row_inc = nullptr;     // [row_subsets*row_width] packed (i<<25 | neg<<24) pairs: see kernels
-    uint32_t* row_inc_rank = nullptr;
-    uint32_t* col_inc = nullptr;
-    uint32_t* col_inc_rank = nullptr;
-};
-
 // Host-side layout of one differential (built like build_common_incidences_generic
 // + build_sector_layout_generic, cuda/cuprym_cyclic.cu:631-790).
 struct PhiHost {
@@ -234,5 +208,32 @@ struct CyclicLayout {
     uint64_t x_cols = 0, z_count = 0, rows = 0, cols = 0;
 };
 void build_cyclic_layout(CyclicLayout& L, const pgw_operator_desc& d);
+
+// Explicit sparse maps (pgw_sparse.cpp): CSR with sorted columns and nonzero
+// canonical coefficients, and its sliced-ELL form for the device (slices of
+// kSellHeight rows; entry k of row r of slice s at ent[2 (slice[s] +
+// k kSellHeight + r % kSellHeight)] = column, coefficient; rows shorter than
+// their slice's widest are padded with (0, 0)).
+constexpr uint64_t kSellHeight = 64;
+struct CsrHost {
+    uint64_t rows = 0, cols = 0;
+    std::vector<uint64_t> ptr;
+    std::vector<uint32_t> col, val;
+};
+struct SellHost {
+    uint64_t rows = 0, cols = 0, nnz = 0;
+    std::vector<uint64_t> slice;
+    std::vector<uint32_t> ent;
+};
+CsrHost csr_transpose(const CsrHost& A);
+SellHost csr_to_sell(const CsrHost& A);
+// The eliminated cyclic operator (or a deformation variant) with Dc, the
+// elimination and P^T Dr P folded in: S = M1^T M2^T M3 M2 M1 (pgw_sparse.cpp).
+struct CyclicMaps {
+    CsrHost m1, m2, m3;
+};
+void build_cyclic_maps(CyclicMaps& out, const CyclicLayout& L, const pgw_operator_desc& d,
+                       uint64_t cols, const uint32_t* dc, const uint32_t* dr, uint32_t rounds,
+                       const pgw_rowmix_pair* pairs);
 
 }  // namespace pgw

@@ -38,46 +38,19 @@ struct DenseSweep {
 void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t* tab_neg,
                         uint32_t k4, const uint32_t* in, uint32_t* out, const uint32_t* scale,
                         uint32_t b, const ModP& m, hipStream_t st);
-void launch_cyclic_fwd(const PhiDevice& d, const uint32_t* in, uint32_t* out,
-                       const uint32_t* scale, uint32_t b, const ModP& m, hipStream_t st);
-void launch_cyclic_trn(const PhiDevice& d, const uint32_t* in, uint32_t* out,
-                       const uint32_t* scale, uint32_t b, const ModP& m, hipStream_t st);
-void launch_elim_y(const PhiDevice& dm, const PhiDevice& d2, const uint32_t* z_off,
-                   const uint16_t* z_kappa, const uint32_t* right_inv, const uint32_t* kbasis,
-                   uint32_t n_v, uint32_t kdim, const uint32_t* z, const uint32_t* t, uint32_t* y,
-                   uint32_t b, const ModP& m, hipStream_t st);
-void launch_elim_rt_h(const PhiDevice& dm, const PhiDevice& d2, const uint32_t* right_inv,
-                      uint32_t n_v, const uint32_t* h, uint32_t* t_adj, uint32_t b,
-                      const ModP& m, hipStream_t st);
-void launch_elim_bt_h(const PhiDevice& d2, uint64_t z_count, const uint32_t* z_rank,
-                      const uint16_t* z_kappa, const uint32_t* kbasis, uint32_t kdim,
-                      const uint32_t* h, uint32_t* z_adj, const uint32_t* scale, uint32_t b,
+// Sliced-ELL sparse map on the device (layout: SellHost, pgw_internal.h).
+struct SellDevice {
+    uint64_t rows = 0, cols = 0, nnz = 0;
+    const uint64_t* slice = nullptr;
+    const uint32_t* ent = nullptr;
+};
+// out (M.rows x b) = M in (M.cols x b) mod p; out must not overlap in
+void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
                       const ModP& m, hipStream_t st);
 
 // elementwise -----------------------------------------------------------------
 void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uint64_t rows,
                        uint32_t b, const ModP& m, hipStream_t st);
-// Deformation variants (dense columns beside an eliminated cyclic operator).
-// Replacement input: scaled[0, base) = Dc-scaled kept columns at their base
-// positions (rows drop0, drop1 zero), scaled[base, base + dcols) = the
-// Dc-scaled dense-column inputs; `in` has base - 2 + dcols rows.
-void launch_complement_scale(const uint32_t* in, const uint32_t* dc, uint32_t* scaled,
-                             uint64_t base, uint32_t drop0, uint32_t drop1, uint32_t dcols,
-                             uint32_t b, const ModP& m, hipStream_t st);
-// out[j] = full[kept j's base position] for j < base - 2
-void launch_complement_gather(const uint32_t* full, uint32_t* out, uint64_t base, uint32_t drop0,
-                              uint32_t drop1, uint32_t b, hipStream_t st);
-// target (rows x b) += (subtract ? -1 : 1) * dense (rows x dcols) * extra (dcols x b)
-void launch_dense_cols_axpy(uint32_t* target, const uint32_t* dense, const uint32_t* extra,
-                            uint64_t rows, uint32_t dcols, uint32_t b, bool subtract,
-                            const ModP& m, hipStream_t st);
-// out[(lead + j) * b + c] = (subtract ? -1 : 1) * (dense^T y)[j][c] * dc[lead + j];
-// `scratch` holds dense_cols_scratch_bytes(dcols, b)
-size_t dense_cols_scratch_bytes(uint32_t dcols, uint32_t b);
-void launch_dense_cols_transpose(const uint32_t* dense, const uint32_t* y, uint64_t rows,
-                                 uint32_t dcols, uint32_t b, const uint32_t* dc, uint64_t lead,
-                                 uint32_t* out, bool subtract, uint64_t* scratch, const ModP& m,
-                                 hipStream_t st);
 // P^T Dr P on a target block in place (P = `rounds` rounds of paired 2x2 maps).
 // `unpaired_row` is the row missing from the last round's pairs (odd row count) or -1.
 void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pair_count,

@@ -172,256 +172,97 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 }
 
 // ===========================================================================
-// plain character-restricted sweeps (reference's cyclic operator)
+// explicit sparse maps (the cyclic operator's chain, pgw_sparse.cpp)
 // ===========================================================================
-
-struct PhiArgs {
-    uint32_t nB, nA, r, row_width, col_width;
-    uint64_t rows, cols;
-    const uint32_t* mu;
-    const uint16_t* wV;
-    const uint16_t* wA0;
-    const uint16_t* wA1;
-    const uint32_t* a0_off;
-    const uint16_t* a0_idx;
-    const uint32_t* a1_off;
-    const uint16_t* a1_idx;
-    const uint32_t* row_off;
-    const uint32_t* row_rank;
-    const uint16_t* row_beta;
-    const uint32_t* col_off;
-    const uint32_t* col_rank;
-    const uint16_t* col_a;
-    const uint32_t* row_inc;
-    const uint32_t* row_inc_rank;
-    const uint32_t* col_inc;
-    const uint32_t* col_inc_rank;
-};
-
-static PhiArgs phi_args(const PhiDevice& d) {
-    PhiArgs a;
-    a.nB = d.nB; a.nA = d.nA; a.r = d.r; a.row_width = d.row_width; a.col_width = d.col_width;
-    a.rows = d.rows; a.cols = d.cols; a.mu = d.mu; a.wV = d.wV; a.wA0 = d.wA0; a.wA1 = d.wA1;
-    a.a0_off = d.a0_off; a.a0_idx = d.a0_idx; a.a1_off = d.a1_off; a.a1_idx = d.a1_idx;
-    a.row_off = d.row_off; a.row_rank = d.row_rank; a.row_beta = d.row_beta;
-    a.col_off = d.col_off; a.col_rank = d.col_rank; a.col_a = d.col_a;
-    a.row_inc = d.row_inc; a.row_inc_rank = d.row_inc_rank;
-    a.col_inc = d.col_inc; a.col_inc_rank = d.col_inc_rank;
-    return a;
-}
-
-// forward: row (s', beta); sum over incidences (i, s = s' + i) and a in the
-// character block w(beta) - w(i) of mu[i][beta][a] * in[colbase(s) + local_a]
-template <int B>
-__global__ __launch_bounds__(256) void k_cyclic_fwd(PhiArgs a, const uint32_t* __restrict__ in,
-                                                    uint32_t* __restrict__ out,
-                                                    const uint32_t* __restrict__ scale, ModP m) {
-    constexpr int RPB = 256 / B;
-    const uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB || row >= a.rows) return;
-    const uint32_t beta = a.row_beta[row];
-    const uint32_t bw = a.wA1[beta];
-    const uint64_t s = a.row_rank[row];
-    uint64_t lo = 0, hi = 0;
-    for (uint32_t pos = 0; pos < a.row_width; ++pos) {
-        const uint32_t code = a.row_inc[s * a.row_width + pos];
-        const uint32_t i = code & 0x7fffffffu;
-        const bool neg = code >> 31;
-        const uint32_t q = (bw + a.r - (a.wV[i] % a.r)) % a.r;
-        const uint32_t start = a.a0_off[q], end = a.a0_off[q + 1];
-        const uint64_t base = a.col_off[a.row_inc_rank[s * a.row_width + pos]];
-        const uint32_t* murow = a.mu + ((uint64_t)i * a.nB + beta) * a.nA;
-        for (uint32_t idx = start; idx < end; ++idx) {
-            uint32_t t = murow[a.a0_idx[idx]];
-            if (t == 0) continue;
-            if (neg) t = m.p - t;
-            uint64_t prod = (uint64_t)t * in[(base + (idx - start)) * B + c];
-            lo += (uint32_t)prod;
-            hi += prod >> 32;
+//
+// out (rows x B) = M in (cols x B) mod p for a sliced-ELL map: the entries of
+// slice s (kSellHeight consecutive rows) are interleaved row by row, so that
+// the 64 lanes of a wavefront read one 512-byte run per step.  CW columns per
+// thread (B / CW threads per row): a 16-byte load gathers four of them.
+// SMALLP (p < 2^16): products < 2^32, summed in u64; otherwise products
+// < 2^62 summed as 64-bit low/high word sums.  Exact for rows of any length
+// below 2^32.
+template <int B, int CW, bool SMALLP>
+__global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ slice,
+                                                   const uint2* __restrict__ ent, uint64_t rows,
+                                                   const uint32_t* __restrict__ in,
+                                                   uint32_t* __restrict__ out, ModP m) {
+    constexpr int G = B / CW;
+    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t row = tid / G;
+    if (row >= rows) return;
+    const uint32_t g = (uint32_t)(tid % G);
+    const uint64_t sl = row / kSellHeight;
+    const uint64_t b0 = slice[sl], width = (slice[sl + 1] - b0) / kSellHeight;
+    const uint2* e = ent + b0 + row % kSellHeight;
+    uint64_t lo[CW], hi[CW];
+#pragma unroll
+    for (int q = 0; q < CW; ++q) lo[q] = hi[q] = 0;
+    for (uint64_t k = 0; k < width; ++k) {
+        const uint2 c = e[k * kSellHeight];
+        const uint32_t* xr = in + (uint64_t)c.x * B + g * CW;
+        uint32_t xv[CW];
+        if constexpr (CW == 4) {
+            const uint4 v = *reinterpret_cast<const uint4*>(xr);
+            xv[0] = v.x; xv[1] = v.y; xv[2] = v.z; xv[3] = v.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < CW; ++q) xv[q] = xr[q];
+        }
+#pragma unroll
+        for (int q = 0; q < CW; ++q) {
+            const uint64_t prod = (uint64_t)c.y * xv[q];
+            if (SMALLP) {
+                lo[q] += prod;
+            } else {
+                lo[q] += (uint32_t)prod;
+                hi[q] += prod >> 32;
+            }
         }
     }
-    uint32_t v = mod_split(hi, lo, m);
-    if (scale) v = mulmod(v, scale[row], m);
-    out[row * B + c] = v;
+#pragma unroll
+    for (int q = 0; q < CW; ++q)
+        out[row * B + g * CW + q] = SMALLP ? mod_u64(lo[q], m) : mod_split(hi[q], lo[q], m);
 }
 
-// transpose: column (s, a); sum over incidences (i, s - i) and beta in the
-// character block w(a) + w(i) of mu[i][beta][a] * in[rowbase(s - i) + local_beta]
-template <int B>
-__global__ __launch_bounds__(256) void k_cyclic_trn(PhiArgs a, const uint32_t* __restrict__ in,
-                                                    uint32_t* __restrict__ out,
-                                                    const uint32_t* __restrict__ scale, ModP m) {
-    constexpr int RPB = 256 / B;
-    const uint64_t col = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB || col >= a.cols) return;
-    const uint32_t av = a.col_a[col];
-    const uint32_t aw = a.wA0[av];
-    const uint64_t s = a.col_rank[col];
-    uint64_t lo = 0, hi = 0;
-    for (uint32_t pos = 0; pos < a.col_width; ++pos) {
-        const uint32_t code = a.col_inc[s * a.col_width + pos];
-        const uint32_t i = code & 0x7fffffffu;
-        const bool neg = code >> 31;
-        const uint32_t q = (aw + a.wV[i]) % a.r;
-        const uint32_t start = a.a1_off[q], end = a.a1_off[q + 1];
-        const uint64_t base = a.row_off[a.col_inc_rank[s * a.col_width + pos]];
-        for (uint32_t idx = start; idx < end; ++idx) {
-            const uint32_t beta = a.a1_idx[idx];
-            uint32_t t = a.mu[((uint64_t)i * a.nB + beta) * a.nA + av];
-            if (t == 0) continue;
-            if (neg) t = m.p - t;
-            uint64_t prod = (uint64_t)t * in[(base + (idx - start)) * B + c];
-            lo += (uint32_t)prod;
-            hi += prod >> 32;
-        }
-    }
-    uint32_t v = mod_split(hi, lo, m);
-    if (scale) v = mulmod(v, scale[col], m);
-    out[col * B + c] = v;
+template <int B, int CW>
+static void sell_b(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
+                   hipStream_t st) {
+    const uint64_t threads = M.rows * (B / CW);
+    const dim3 grid(grid_for(threads)), blk(256);
+    const uint2* ent = reinterpret_cast<const uint2*>(M.ent);
+    if (m.p < 65536u)
+        hipLaunchKernelGGL((k_sell_spmm<B, CW, true>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
+    else
+        hipLaunchKernelGGL((k_sell_spmm<B, CW, false>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
 }
 
-void launch_cyclic_fwd(const PhiDevice& d, const uint32_t* in, uint32_t* out,
-                       const uint32_t* scale, uint32_t b, const ModP& m, hipStream_t st) {
-    if (d.rows == 0) return;
-    PhiArgs a = phi_args(d);
-#define PGW_CALL(BB)                                                                         \
-    hipLaunchKernelGGL(k_cyclic_fwd<BB>, dim3(grid_for(d.rows, 256 / BB)), dim3(256), 0, st, \
-                       a, in, out, scale, m)
-    PGW_SWITCH_B(b, PGW_CALL)
-#undef PGW_CALL
-    PGW_CHECK_LAUNCH("cyclic forward sweep");
-}
-
-void launch_cyclic_trn(const PhiDevice& d, const uint32_t* in, uint32_t* out,
-                       const uint32_t* scale, uint32_t b, const ModP& m, hipStream_t st) {
-    if (d.cols == 0) return;
-    PhiArgs a = phi_args(d);
-#define PGW_CALL(BB)                                                                         \
-    hipLaunchKernelGGL(k_cyclic_trn<BB>, dim3(grid_for(d.cols, 256 / BB)), dim3(256), 0, st, \
-                       a, in, out, scale, m)
-    PGW_SWITCH_B(b, PGW_CALL)
-#undef PGW_CALL
-    PGW_CHECK_LAUNCH("cyclic transpose sweep");
-}
-
-// ===========================================================================
-// elimination bridge (cuda/cyclic_cuda_helpers.h:718-895)
-// ===========================================================================
-
-// y[s,a] = sum_kappa Bk[a][kappa] z[s,kappa] - sum_beta R[a][beta] t[s,beta]
-template <int B>
-__global__ __launch_bounds__(256) void k_elim_y(
-    uint64_t y_rows, const uint32_t* __restrict__ y_rank, const uint16_t* __restrict__ y_a,
-    const uint32_t* __restrict__ z_off, const uint16_t* __restrict__ z_kappa,
-    const uint32_t* __restrict__ t_off, const uint16_t* __restrict__ t_beta,
-    const uint32_t* __restrict__ right_inv, const uint32_t* __restrict__ kbasis, uint32_t n_v,
-    uint32_t kdim, const uint32_t* __restrict__ z, const uint32_t* __restrict__ t,
-    uint32_t* __restrict__ y, ModP m) {
-    constexpr int RPB = 256 / B;
-    const uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB || row >= y_rows) return;
-    const uint32_t s = y_rank[row], a = y_a[row];
-    uint64_t plo = 0, phi = 0, mlo = 0, mhi = 0;
-    for (uint32_t idx = z_off[s]; idx < z_off[s + 1]; ++idx) {
-        uint64_t prod = (uint64_t)kbasis[(uint64_t)a * kdim + z_kappa[idx]] * z[(uint64_t)idx * B + c];
-        plo += (uint32_t)prod;
-        phi += prod >> 32;
-    }
-    for (uint32_t idx = t_off[s]; idx < t_off[s + 1]; ++idx) {
-        uint64_t prod = (uint64_t)right_inv[(uint64_t)a * n_v + t_beta[idx]] * t[(uint64_t)idx * B + c];
-        mlo += (uint32_t)prod;
-        mhi += prod >> 32;
-    }
-    uint32_t pv = mod_split(phi, plo, m), mv = mod_split(mhi, mlo, m);
-    y[row * B + c] = addmod(pv, mv ? m.p - mv : 0u, m.p);
-}
-
-// t_adj[s,beta] = sum_a R[a][beta] h[s,a]
-template <int B>
-__global__ __launch_bounds__(256) void k_elim_rt_h(
-    uint64_t t_rows, const uint32_t* __restrict__ t_rank, const uint16_t* __restrict__ t_beta,
-    const uint32_t* __restrict__ y_off, const uint16_t* __restrict__ y_a,
-    const uint32_t* __restrict__ right_inv, uint32_t n_v, const uint32_t* __restrict__ h,
-    uint32_t* __restrict__ t_adj, ModP m) {
-    constexpr int RPB = 256 / B;
-    const uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB || row >= t_rows) return;
-    const uint32_t s = t_rank[row], beta = t_beta[row];
-    uint64_t lo = 0, hi = 0;
-    for (uint32_t idx = y_off[s]; idx < y_off[s + 1]; ++idx) {
-        uint64_t prod = (uint64_t)right_inv[(uint64_t)y_a[idx] * n_v + beta] * h[(uint64_t)idx * B + c];
-        lo += (uint32_t)prod;
-        hi += prod >> 32;
-    }
-    t_adj[row * B + c] = mod_split(hi, lo, m);
-}
-
-// z_adj[s,kappa] = scale * sum_a Bk[a][kappa] h[s,a]
-template <int B>
-__global__ __launch_bounds__(256) void k_elim_bt_h(
-    uint64_t z_rows, const uint32_t* __restrict__ z_rank, const uint16_t* __restrict__ z_kappa,
-    const uint32_t* __restrict__ y_off, const uint16_t* __restrict__ y_a,
-    const uint32_t* __restrict__ kbasis, uint32_t kdim, const uint32_t* __restrict__ h,
-    uint32_t* __restrict__ z_adj, const uint32_t* __restrict__ scale, ModP m) {
-    constexpr int RPB = 256 / B;
-    const uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB || row >= z_rows) return;
-    const uint32_t s = z_rank[row], kappa = z_kappa[row];
-    uint64_t lo = 0, hi = 0;
-    for (uint32_t idx = y_off[s]; idx < y_off[s + 1]; ++idx) {
-        uint64_t prod = (uint64_t)kbasis[(uint64_t)y_a[idx] * kdim + kappa] * h[(uint64_t)idx * B + c];
-        lo += (uint32_t)prod;
-        hi += prod >> 32;
-    }
-    uint32_t v = mod_split(hi, lo, m);
-    if (scale) v = mulmod(v, scale[row], m);
-    z_adj[row * B + c] = v;
-}
-
-void launch_elim_y(const PhiDevice& dm, const PhiDevice& d2, const uint32_t* z_off,
-                   const uint16_t* z_kappa, const uint32_t* right_inv, const uint32_t* kbasis,
-                   uint32_t n_v, uint32_t kdim, const uint32_t* z, const uint32_t* t, uint32_t* y,
-                   uint32_t b, const ModP& m, hipStream_t st) {
-    if (d2.cols == 0) return;
-#define PGW_CALL(BB)                                                                           \
-    hipLaunchKernelGGL(k_elim_y<BB>, dim3(grid_for(d2.cols, 256 / BB)), dim3(256), 0, st,      \
-                       d2.cols, d2.col_rank, d2.col_a, z_off, z_kappa, dm.row_off, dm.row_beta, \
-                       right_inv, kbasis, n_v, kdim, z, t, y, m)
-    PGW_SWITCH_B(b, PGW_CALL)
-#undef PGW_CALL
-    PGW_CHECK_LAUNCH("elimination y");
-}
-
-void launch_elim_rt_h(const PhiDevice& dm, const PhiDevice& d2, const uint32_t* right_inv,
-                      uint32_t n_v, const uint32_t* h, uint32_t* t_adj, uint32_t b,
+void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
                       const ModP& m, hipStream_t st) {
-    if (dm.rows == 0) return;
-#define PGW_CALL(BB)                                                                          \
-    hipLaunchKernelGGL(k_elim_rt_h<BB>, dim3(grid_for(dm.rows, 256 / BB)), dim3(256), 0, st,  \
-                       dm.rows, dm.row_rank, dm.row_beta, d2.col_off, d2.col_a, right_inv, n_v, \
-                       h, t_adj, m)
-    PGW_SWITCH_B(b, PGW_CALL)
-#undef PGW_CALL
-    PGW_CHECK_LAUNCH("elimination R^T h");
-}
-
-void launch_elim_bt_h(const PhiDevice& d2, uint64_t z_count, const uint32_t* z_rank,
-                      const uint16_t* z_kappa, const uint32_t* kbasis, uint32_t kdim,
-                      const uint32_t* h, uint32_t* z_adj, const uint32_t* scale, uint32_t b,
-                      const ModP& m, hipStream_t st) {
-    if (z_count == 0) return;
-#define PGW_CALL(BB)                                                                          \
-    hipLaunchKernelGGL(k_elim_bt_h<BB>, dim3(grid_for(z_count, 256 / BB)), dim3(256), 0, st,  \
-                       z_count, z_rank, z_kappa, d2.col_off, d2.col_a, kbasis, kdim, h, z_adj, \
-                       scale, m)
-    PGW_SWITCH_B(b, PGW_CALL)
-#undef PGW_CALL
-    PGW_CHECK_LAUNCH("elimination B^T h");
+    if (M.rows == 0) return;
+    require(M.rows * b / b == M.rows && M.rows < (1ULL << 40), "sparse map too large");
+    // 16-byte gathers need 16-byte aligned rows
+    const bool vec = b % 4 == 0 && (reinterpret_cast<uintptr_t>(in) % 16) == 0;
+    switch (b) {
+        case 1: sell_b<1, 1>(M, in, out, m, st); break;
+        case 2: sell_b<2, 2>(M, in, out, m, st); break;
+        case 3: sell_b<3, 3>(M, in, out, m, st); break;
+        case 4: if (vec) sell_b<4, 4>(M, in, out, m, st); else sell_b<4, 1>(M, in, out, m, st); break;
+        case 5: sell_b<5, 1>(M, in, out, m, st); break;
+        case 6: sell_b<6, 2>(M, in, out, m, st); break;
+        case 7: sell_b<7, 1>(M, in, out, m, st); break;
+        case 8: if (vec) sell_b<8, 4>(M, in, out, m, st); else sell_b<8, 1>(M, in, out, m, st); break;
+        case 9: sell_b<9, 3>(M, in, out, m, st); break;
+        case 10: sell_b<10, 2>(M, in, out, m, st); break;
+        case 11: sell_b<11, 1>(M, in, out, m, st); break;
+        case 12: if (vec) sell_b<12, 4>(M, in, out, m, st); else sell_b<12, 1>(M, in, out, m, st); break;
+        case 13: sell_b<13, 1>(M, in, out, m, st); break;
+        case 14: sell_b<14, 2>(M, in, out, m, st); break;
+        case 15: sell_b<15, 3>(M, in, out, m, st); break;
+        case 16: if (vec) sell_b<16, 4>(M, in, out, m, st); else sell_b<16, 1>(M, in, out, m, st); break;
+        default: throw Error(PGW_ERR_UNSUPPORTED, "block width must be in 1..16");
+    }
+    PGW_CHECK_LAUNCH("sparse map");
 }
 
 // ===========================================================================
@@ -444,162 +285,6 @@ void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uin
     unsigned g = (unsigned)std::min<uint64_t>(grid_for(total), 65536);
     hipLaunchKernelGGL(k_scale_rows, dim3(g), dim3(256), 0, st, in, s, out, total, b, m);
     PGW_CHECK_LAUNCH("scale rows");
-}
-
-// ===========================================================================
-// deformation variants: dense columns beside the eliminated cyclic operator
-// (cyclic_dense_columns_*_kernel, cyclic_insert_complement_scaled_kernel,
-// cyclic_gather_complement_kernel: cuda/cyclic_cuda_helpers.h:1210-1351)
-// ===========================================================================
-
-// base position of kept column j (the base columns in order, drop0 < drop1 left out)
-__device__ __forceinline__ uint64_t kept_to_base(uint64_t j, uint32_t drop0, uint32_t drop1) {
-    if (j >= drop0) ++j;
-    if (j >= drop1) ++j;
-    return j;
-}
-
-__global__ __launch_bounds__(256) void k_complement_scale(const uint32_t* __restrict__ in,
-                                                          const uint32_t* __restrict__ dc,
-                                                          uint32_t* __restrict__ scaled, uint64_t base,
-                                                          uint32_t drop0, uint32_t drop1, uint64_t total,
-                                                          uint32_t b, ModP m) {
-    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * 256) {
-        const uint64_t r = e / b, c = e % b;
-        uint64_t src;  // input row feeding scaled row r
-        if (r < base) {
-            if (r == drop0 || r == drop1) {
-                scaled[e] = 0;
-                continue;
-            }
-            src = r - (r > drop0 ? 1 : 0) - (r > drop1 ? 1 : 0);
-        } else {
-            src = r - 2;  // dense-column inputs follow the base - 2 kept columns
-        }
-        scaled[e] = mulmod(in[src * b + c], dc[src], m);
-    }
-}
-
-void launch_complement_scale(const uint32_t* in, const uint32_t* dc, uint32_t* scaled,
-                             uint64_t base, uint32_t drop0, uint32_t drop1, uint32_t dcols,
-                             uint32_t b, const ModP& m, hipStream_t st) {
-    const uint64_t total = (base + dcols) * b;
-    if (total == 0) return;
-    unsigned g = (unsigned)std::min<uint64_t>(grid_for(total), 65536);
-    hipLaunchKernelGGL(k_complement_scale, dim3(g), dim3(256), 0, st, in, dc, scaled, base, drop0,
-                       drop1, total, b, m);
-    PGW_CHECK_LAUNCH("complement scale");
-}
-
-__global__ __launch_bounds__(256) void k_complement_gather(const uint32_t* __restrict__ full,
-                                                           uint32_t* __restrict__ out, uint64_t total,
-                                                           uint32_t drop0, uint32_t drop1, uint32_t b) {
-    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * 256) {
-        const uint64_t j = e / b, c = e % b;
-        out[e] = full[kept_to_base(j, drop0, drop1) * b + c];
-    }
-}
-
-void launch_complement_gather(const uint32_t* full, uint32_t* out, uint64_t base, uint32_t drop0,
-                              uint32_t drop1, uint32_t b, hipStream_t st) {
-    const uint64_t total = (base - 2) * b;
-    if (total == 0) return;
-    unsigned g = (unsigned)std::min<uint64_t>(grid_for(total), 65536);
-    hipLaunchKernelGGL(k_complement_gather, dim3(g), dim3(256), 0, st, full, out, total, drop0, drop1, b);
-    PGW_CHECK_LAUNCH("complement gather");
-}
-
-// one thread per target entry: sum_j dense[r][j] * extra[j][c], each product
-// reduced (any dcols), then added to or subtracted from the target entry
-__global__ __launch_bounds__(256) void k_dense_cols_axpy(uint32_t* __restrict__ target,
-                                                         const uint32_t* __restrict__ dense,
-                                                         const uint32_t* __restrict__ extra,
-                                                         uint64_t total, uint32_t dcols, uint32_t b,
-                                                         int subtract, ModP m) {
-    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * 256) {
-        const uint64_t r = e / b, c = e % b;
-        uint64_t sum = 0;  // dcols terms below 2^31 each
-        for (uint32_t j = 0; j < dcols; ++j)
-            sum += mulmod(dense[r * dcols + j], extra[(uint64_t)j * b + c], m);
-        const uint32_t t = mod_u64(sum, m);
-        const uint32_t x = target[e];
-        target[e] = subtract ? addmod(x, t ? m.p - t : 0u, m.p) : addmod(x, t, m.p);
-    }
-}
-
-void launch_dense_cols_axpy(uint32_t* target, const uint32_t* dense, const uint32_t* extra,
-                            uint64_t rows, uint32_t dcols, uint32_t b, bool subtract,
-                            const ModP& m, hipStream_t st) {
-    const uint64_t total = rows * b;
-    if (total == 0) return;
-    unsigned g = (unsigned)std::min<uint64_t>(grid_for(total), 65536);
-    hipLaunchKernelGGL(k_dense_cols_axpy, dim3(g), dim3(256), 0, st, target, dense, extra, total,
-                       dcols, b, subtract ? 1 : 0, m);
-    PGW_CHECK_LAUNCH("dense columns axpy");
-}
-
-// dense^T y in two passes and without atomics: each of at most kDenseTBlocks
-// workgroups sums its rows (a grid-stride walk) for every (j, c) pair with
-// reduced products in u64 (exact for any row count below 2^32), the second
-// pass adds the partials, reduces once, signs and scales.
-constexpr uint32_t kDenseTBlocks = 256;
-
-size_t dense_cols_scratch_bytes(uint32_t dcols, uint32_t b) {
-    return (size_t)kDenseTBlocks * dcols * b * sizeof(uint64_t);
-}
-
-__global__ __launch_bounds__(256) void k_dense_cols_t_partial(const uint32_t* __restrict__ dense,
-                                                              const uint32_t* __restrict__ y,
-                                                              uint64_t rows, uint32_t dcols, uint32_t b,
-                                                              uint64_t* __restrict__ partial, ModP m) {
-    __shared__ uint64_t s[256];
-    const uint32_t np = dcols * b;  // (j, c) pairs, np <= 256
-    const uint32_t rl = 256 / np;   // rows walked side by side
-    const uint32_t t = threadIdx.x, pair = t % np, lane = t / np;
-    uint64_t acc = 0;
-    if (lane < rl) {
-        const uint32_t j = pair / b, c = pair % b;
-        for (uint64_t r = (uint64_t)blockIdx.x * rl + lane; r < rows; r += (uint64_t)gridDim.x * rl)
-            acc += mulmod(dense[r * dcols + j], y[r * b + c], m);
-    }
-    s[t] = acc;
-    __syncthreads();
-    if (t < np) {
-        uint64_t sum = 0;
-        for (uint32_t q = 0; q < rl; ++q) sum += s[q * np + t];
-        partial[(uint64_t)blockIdx.x * np + t] = sum;
-    }
-}
-
-__global__ void k_dense_cols_t_finish(const uint64_t* __restrict__ partial, uint32_t blocks,
-                                      uint32_t dcols, uint32_t b, const uint32_t* __restrict__ dc,
-                                      uint64_t lead, uint32_t* __restrict__ out, int subtract, ModP m) {
-    const uint32_t np = dcols * b, t = threadIdx.x;
-    if (t >= np) return;
-    uint64_t sum = 0;  // blocks * (rows / blocks) * 2^31 < 2^63
-    for (uint32_t q = 0; q < blocks; ++q) sum += partial[(uint64_t)q * np + t];
-    uint32_t v = mod_u64(sum, m);
-    if (subtract && v) v = m.p - v;
-    const uint32_t j = t / b, c = t % b;
-    out[(lead + j) * b + c] = mulmod(v, dc[lead + j], m);
-}
-
-void launch_dense_cols_transpose(const uint32_t* dense, const uint32_t* y, uint64_t rows,
-                                 uint32_t dcols, uint32_t b, const uint32_t* dc, uint64_t lead,
-                                 uint32_t* out, bool subtract, uint64_t* scratch, const ModP& m,
-                                 hipStream_t st) {
-    const uint32_t np = dcols * b;
-    require(np >= 1 && np <= 256, "dense columns x block width must be at most 256");
-    const uint32_t rl = 256 / np;
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kDenseTBlocks, (rows + rl - 1) / rl));
-    hipLaunchKernelGGL(k_dense_cols_t_partial, dim3(blocks), dim3(256), 0, st, dense, y, rows, dcols,
-                       b, scratch, m);
-    hipLaunchKernelGGL(k_dense_cols_t_finish, dim3(1), dim3(256), 0, st, scratch, blocks, dcols, b,
-                       dc, lead, out, subtract ? 1 : 0, m);
-    PGW_CHECK_LAUNCH("dense columns transpose");
 }
 
 // v mod p for v < 2^63.  M31 (p = 2^31 - 1): 2^31 == 1, so two folds of the
@@ -704,12 +389,20 @@ void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64
 // Gram projection: packed upper triangle of X^T Y (and Y^T Y or X^T X)
 // ===========================================================================
 //
-// Each workgroup stages 64-row chunks of X and Y in LDS; thread (i,j) keeps
-// 64-bit low/high word sums of the products (each < 2^62), so partial sums are
-// exact for any N < 2^32 rows; a second kernel adds the partials and reduces.
+// Partial pass: each workgroup stages 64-row chunks of X and Y in LDS; its
+// threads are (pair, row lane) with pair = ut(i, j) = i(2b-i+1)/2 + (j-i) and
+// 256 / (b(b+1)/2) row lanes, so every thread works for any b (b = 4: 25 lanes
+// of the 10 pairs).  Each keeps 64-bit sums of the low and high 32-bit words
+// of its products (each product < 2^62), lanes are added in LDS, and the
+// workgroup's sums land at part[(mat * P + pair) * nblocks + block].  Every
+// partial and every sum of them stays below rows * 2^32 < 2^64 for rows <
+// 2^32, so all of it is exact.  Finish pass: one workgroup per output entry
+// adds that entry's partials (coalesced, then an LDS tree), reduces once and
+// writes the packed value.
 
 constexpr int kGramRows = 64;
 constexpr unsigned kGramGrid = 1024;
+constexpr uint64_t kGramRowsPerBlock = 256; // least rows per partial workgroup
 
 size_t gram_scratch_bytes(uint32_t b) {
     return (size_t)kGramGrid * 2 * 256 * 2 * sizeof(uint64_t);
@@ -719,30 +412,39 @@ template <int B>
 __global__ __launch_bounds__(256) void k_gram_partial(const uint32_t* __restrict__ x,
                                                       const uint32_t* __restrict__ y, uint64_t rows,
                                                       int mode, uint64_t* __restrict__ part) {
+    constexpr int P = B * (B + 1) / 2;
+    constexpr int L = 256 / P;  // row lanes (P <= 136)
     __shared__ uint32_t sx[kGramRows * B];
     __shared__ uint32_t sy[kGramRows * B];
+    __shared__ uint64_t red[4][L * P];
     const int t = threadIdx.x;
-    const int i = t / 16, j = t % 16;
-    const bool active = i < B && j < B && i <= j;
+    const bool active = t < L * P;
+    const int pr = t % P, lane = t / P;
+    int i = 0, rem = pr;
+    while (rem >= B - i) {
+        rem -= B - i;
+        ++i;
+    }
+    const int j = i + rem;
     uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
     for (uint64_t r0 = (uint64_t)blockIdx.x * kGramRows; r0 < rows;
          r0 += (uint64_t)gridDim.x * kGramRows) {
         const uint64_t nrow = min((uint64_t)kGramRows, rows - r0);
         __syncthreads();
         for (int e = t; e < kGramRows * B; e += 256) {
-            bool ok = (uint64_t)(e / B) < nrow;
+            const bool ok = (uint64_t)(e / B) < nrow;
             sx[e] = ok ? x[r0 * B + e] : 0u;
             sy[e] = ok ? y[r0 * B + e] : 0u;
         }
         __syncthreads();
         if (active) {
-            for (int r = 0; r < kGramRows; ++r) {
-                uint64_t p0 = (uint64_t)sx[r * B + i] * sy[r * B + j];
+            for (int r = lane; r < kGramRows; r += L) {
+                const uint64_t p0 = (uint64_t)sx[r * B + i] * sy[r * B + j];
                 lo0 += (uint32_t)p0;
                 hi0 += p0 >> 32;
                 if (mode) {
-                    uint64_t p1 = mode == 1 ? (uint64_t)sy[r * B + i] * sy[r * B + j]
-                                            : (uint64_t)sx[r * B + i] * sx[r * B + j];
+                    const uint64_t p1 = mode == 1 ? (uint64_t)sy[r * B + i] * sy[r * B + j]
+                                                  : (uint64_t)sx[r * B + i] * sx[r * B + j];
                     lo1 += (uint32_t)p1;
                     hi1 += p1 >> 32;
                 }
@@ -750,40 +452,60 @@ __global__ __launch_bounds__(256) void k_gram_partial(const uint32_t* __restrict
         }
     }
     if (active) {
-        uint64_t* o = part + ((uint64_t)blockIdx.x * 2 * 256 + (uint64_t)(i * 16 + j)) * 2;
-        o[0] = lo0;
-        o[1] = hi0;
-        o[512] = lo1;
-        o[513] = hi1;
+        red[0][t] = lo0;
+        red[1][t] = hi0;
+        red[2][t] = lo1;
+        red[3][t] = hi1;
+    }
+    __syncthreads();
+    if (t < P) {
+        uint64_t a[4] = {0, 0, 0, 0};
+        for (int l = 0; l < L; ++l)
+            for (int q = 0; q < 4; ++q) a[q] += red[q][l * P + t];
+        const uint64_t nb = gridDim.x;
+        uint64_t* o = part + ((uint64_t)t * nb + blockIdx.x) * 2;
+        o[0] = a[0];
+        o[1] = a[1];
+        if (mode) {
+            uint64_t* o1 = part + ((uint64_t)(P + t) * nb + blockIdx.x) * 2;
+            o1[0] = a[2];
+            o1[1] = a[3];
+        }
     }
 }
 
-// With `pos` set, out0/out1 are offset by ut * (*pos + pos_off) entries: the
-// sequence position is read on the device, so a captured graph of a whole
-// round can be replayed unchanged.
+// One workgroup per output entry (mat * P + pair).  With `pos` set, out0/out1
+// are offset by ut * (*pos + pos_off) entries: the sequence position is read
+// on the device, so a captured graph of a whole round replays unchanged.
 __global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict__ part,
-                                                     unsigned nblocks, uint32_t b, int mode,
+                                                     unsigned nblocks, uint32_t P,
                                                      uint32_t* __restrict__ out0,
                                                      uint32_t* __restrict__ out1, ModP m,
                                                      const uint32_t* __restrict__ pos,
                                                      uint32_t pos_off) {
-    const int t = threadIdx.x;
-    const int i = t / 16, j = t % 16;
-    if (i >= (int)b || j >= (int)b || i > j) return;
-    if (pos) {
-        const uint64_t shift = (uint64_t)(b * (b + 1) / 2) * (*pos + pos_off);
-        out0 += shift;
-        if (out1) out1 += shift;
+    __shared__ uint64_t s_lo[256], s_hi[256];
+    const uint32_t e = blockIdx.x, t = threadIdx.x;
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t g = t; g < nblocks; g += 256) {
+        const uint64_t* o = part + ((uint64_t)e * nblocks + g) * 2;
+        lo += o[0];
+        hi += o[1];
     }
-    const int ut = i * (2 * (int)b - i + 1) / 2 + (j - i);
-    for (int mat = 0; mat < (mode ? 2 : 1); ++mat) {
-        uint64_t lo = 0, hi = 0;
-        for (unsigned g = 0; g < nblocks; ++g) {
-            const uint64_t* o = part + (((uint64_t)g * 2 + mat) * 256 + t) * 2;
-            lo += o[0];
-            hi += o[1];
+    s_lo[t] = lo;
+    s_hi[t] = hi;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s_lo[t] += s_lo[t + w];
+            s_hi[t] += s_hi[t + w];
         }
-        (mat ? out1 : out0)[ut] = mod_split(hi, lo, m);
+        __syncthreads();
+    }
+    if (t == 0) {
+        const uint32_t mat = e / P, pr = e % P;
+        uint32_t* out = mat ? out1 : out0;
+        if (pos) out += (uint64_t)P * (*pos + pos_off);
+        out[pr] = mod_split(s_hi[0], s_lo[0], m);
     }
 }
 
@@ -796,19 +518,29 @@ void launch_advance_pos(uint32_t* pos, uint32_t by, hipStream_t st) {
     PGW_CHECK_LAUNCH("advance position");
 }
 
+// the two Gram kernels over at most `max_grid` workgroups (partials in `part`)
+static void gram_launch(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
+                        uint32_t* out0, uint32_t* out1, uint64_t* part, unsigned max_grid,
+                        const ModP& m, hipStream_t st, const uint32_t* pos, uint32_t pos_off) {
+    // low words: rows * 2^32 < 2^64 requires rows < 2^32
+    require(rows < (1ULL << 32), "Gram projection supports fewer than 2^32 rows");
+    require(mode == 0 || out1 != nullptr, "Gram projection of two products needs both outputs");
+    const unsigned grid = (unsigned)std::min<uint64_t>(
+        max_grid, std::max<uint64_t>(1, (rows + kGramRowsPerBlock - 1) / kGramRowsPerBlock));
+    const uint32_t P = b * (b + 1) / 2;
+#define PGW_CALL(BB)                                                                           \
+    hipLaunchKernelGGL(k_gram_partial<BB>, dim3(grid), dim3(256), 0, st, x, y, rows, mode, part)
+    PGW_SWITCH_B(b, PGW_CALL)
+#undef PGW_CALL
+    hipLaunchKernelGGL(k_gram_finish, dim3(P * (mode ? 2 : 1)), dim3(256), 0, st, part, grid, P,
+                       out0, out1, m, pos, pos_off);
+    PGW_CHECK_LAUNCH("gram");
+}
+
 void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
                  uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
                  hipStream_t st, const uint32_t* pos, uint32_t pos_off) {
-    // low words: rows * 2^32 < 2^64 requires rows < 2^32
-    require(rows < (1ULL << 32), "Gram projection supports fewer than 2^32 rows");
-    unsigned grid = (unsigned)std::min<uint64_t>(kGramGrid, std::max<uint64_t>(1, (rows + kGramRows - 1) / kGramRows));
-#define PGW_CALL(BB)                                                                           \
-    hipLaunchKernelGGL(k_gram_partial<BB>, dim3(grid), dim3(256), 0, st, x, y, rows, mode, scratch)
-    PGW_SWITCH_B(b, PGW_CALL)
-#undef PGW_CALL
-    hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(256), 0, st, scratch, grid, b, mode, out0,
-                       out1, m, pos, pos_off);
-    PGW_CHECK_LAUNCH("gram");
+    gram_launch(x, y, rows, b, mode, out0, out1, scratch, kGramGrid, m, st, pos, pos_off);
 }
 
 // ===========================================================================

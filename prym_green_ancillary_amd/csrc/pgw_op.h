@@ -85,7 +85,7 @@ struct pgw_op {
     using ModP = pgw::ModP;
     using DeviceArena = pgw::DeviceArena;
     using MfmaTable = pgw::MfmaTable;
-    using PhiDevice = pgw::PhiDevice;
+    using SellDevice = pgw::SellDevice;
     pgw_ctx* ctx = nullptr;
     uint32_t kind = 0, p = 0, b = 0, method = PGW_METHOD_PLAIN;
     ModP m{};
@@ -113,27 +113,13 @@ struct pgw_op {
     int8_t* d_lp_src = nullptr;
     int8_t* d_lp_mid = nullptr;
     uint32_t* d_inc[2] = {nullptr, nullptr};  // incidence tables: forward, transpose sweeps
-    // cyclic eliminated
-    PhiDevice dm, d2;
-    uint32_t* d_zoff = nullptr;
-    uint32_t* d_zrank = nullptr;
-    uint16_t* d_zkappa = nullptr;
-    uint32_t* d_R = nullptr;
-    uint32_t* d_Bk = nullptr;
-    uint32_t n_v = 0, kdim = 0;
-    uint64_t x_cols = 0, z_count = 0;
-    uint32_t* d_t = nullptr;
+    // cyclic eliminated and its deformation variants: S = M1^T M2^T M3 M2 M1
+    // (explicit sparse maps, pgw_sparse.cpp); d_y holds y' (M1's rows), d_t2
+    // the mixed target block
+    SellDevice m1, m2, m3, m2t, m1t;
+    uint64_t y_rows = 0;
     uint32_t* d_y = nullptr;
-    uint32_t* d_h = nullptr;
-    uint32_t* d_tadj = nullptr;
-    // deformation variants (augmented / replacement) of the cyclic operator
-    uint64_t base_cols = 0;              // columns of the eliminated base operator
-    uint32_t dense_cols = 0, drop0 = 0, drop1 = 0;
-    uint32_t* d_dense = nullptr;         // rows x dense_cols
-    uint32_t* d_base_dc = nullptr;       // Dc at the base positions (replacement: 0 at the drops)
-    uint32_t* d_base_negdc = nullptr;    // p - d_base_dc
-    uint32_t* d_base_adj = nullptr;      // replacement: F^T of the target, base_cols x b
-    uint64_t* d_dense_scratch = nullptr; // partial sums of dense^T (target)
+    uint32_t* d_t2 = nullptr;
     // timing of the sweep kernels (pgw_bench_apply)
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> sweep_events;
