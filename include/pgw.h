@@ -185,6 +185,18 @@ int pgw_apply_forward(pgw_op* op, const uint32_t* d_in, uint32_t* d_out, void* s
 int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_t* d_out,
                    void* stream);
 
+/* pgw_gram_upper without an operator and without any allocation: d_out
+ * (b(b+1)/2 values) = packed upper triangle of d_x^T d_y mod `prime` for two
+ * rows x block device blocks (1 <= block <= 16, rows < 2^32).  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its partial sums live in a fixed 2 MiB device array per GPU
+ * (set up by pgw_ctx_create, held until the process ends); calls that
+ * overlap on one GPU share that array and race, so serialise them. */
+int pgw_gram_upper_noalloc(pgw_ctx* ctx, uint32_t prime, uint32_t block, uint64_t rows,
+                           const uint32_t* d_x, const uint32_t* d_y, uint32_t* d_out,
+                           void* stream);
+
 /* Progress callback: terms done, terms total. */
 typedef void (*pgw_progress_fn)(uint64_t done, uint64_t total, void* user);
 

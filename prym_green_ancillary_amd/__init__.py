@@ -104,7 +104,7 @@ EXPORTED = [
     "pgw_server_submit", "pgw_server_wait", "pgw_server_result", "pgw_server_stats",
     "pgw_server_destroy", "pgw_job_footprint", "pgw_rank_create", "pgw_rank_step",
     "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
-    "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write",
+    "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_gram_upper_noalloc",
 ]
 
 
@@ -147,6 +147,7 @@ def lib():
         "pgw_apply_square": (I, [vp, vp, vp, vp]),
         "pgw_apply_forward": (I, [vp, vp, vp, vp]),
         "pgw_gram_upper": (I, [vp, vp, vp, vp, vp]),
+        "pgw_gram_upper_noalloc": (I, [vp, u32, u32, u64, vp, vp, vp, vp]),
         "pgw_sequence_run": (I, [vp, vp, u64, vp, vp, ctypes.POINTER(u64), PROGRESS_FN, vp]),
         "pgw_kernel_vectors": (I, [vp, vp, vp, u64, vp, vp]),
         "pgw_bench_apply": (I, [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_double),
@@ -302,6 +303,12 @@ class Context:
         total = ctypes.c_uint64()
         _check(lib().pgw_ctx_device_info(self.h, name, 256, ctypes.byref(total)))
         return name.value.decode(), total.value
+
+    def gram_upper_noalloc(self, prime: int, block: int, rows: int, d_x: int, d_y: int,
+                           d_out: int, stream: int | None = None):
+        """Packed upper triangle of X^T Y mod p (pgw_gram_upper_noalloc): no
+        operator and no device allocation; calls on one GPU must not overlap."""
+        _check(lib().pgw_gram_upper_noalloc(self.h, prime, block, rows, d_x, d_y, d_out, stream))
 
     def close(self):
         if getattr(self, "h", None):

@@ -210,6 +210,7 @@ int pgw_ctx_create(int device, pgw_ctx** out) {
             throw Error(PGW_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
         }
         upload_binomials(device);
+        gram_noalloc_setup(device);
         std::vector<uint64_t> binom(64 * 64, 0);
         for (uint32_t a = 0; a < 64; ++a)
             for (uint32_t b = 0; b <= a; ++b) {
@@ -515,6 +516,19 @@ int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_
         PGW_HIP(hipSetDevice(op->ctx->device));
         launch_gram(d_x, d_y, op->cols, op->b, 0, d_out, nullptr, op->d_gram, op->m,
                     pick_stream(op, stream));
+    });
+}
+
+int pgw_gram_upper_noalloc(pgw_ctx* ctx, uint32_t prime, uint32_t block, uint64_t rows,
+                           const uint32_t* d_x, const uint32_t* d_y, uint32_t* d_out, void* stream) {
+    return guarded([&] {
+        require(ctx && d_x && d_y && d_out, "null context or block");
+        require(prime > 2 && prime < (1u << 31), "prime must satisfy 2 < p < 2^31");
+        require(block >= 1 && block <= 16, "block width must be in 1..16");
+        require(rows < (1ULL << 32), "Gram projection supports fewer than 2^32 rows");
+        PGW_HIP(hipSetDevice(ctx->device));
+        launch_gram_noalloc(ctx->device, d_x, d_y, rows, block, d_out, make_modp(prime),
+                            stream ? (hipStream_t)stream : ctx->stream);
     });
 }
 

@@ -64,6 +64,14 @@ size_t gram_scratch_bytes(uint32_t b);
 void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
                  uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
                  hipStream_t st, const uint32_t* pos = nullptr, uint32_t pos_off = 0);
+// Allocation-free Gram (mode 0 only, out = X^T Y packed): partials in a
+// fixed device array of gram_noalloc_scratch_bytes() (2 MiB) per GPU.
+// gram_noalloc_setup resolves that array's address on `device` (call it with
+// the device current, once, before the first launch on it).
+size_t gram_noalloc_scratch_bytes();
+void gram_noalloc_setup(int device);
+void launch_gram_noalloc(int device, const uint32_t* x, const uint32_t* y, uint64_t rows,
+                         uint32_t b, uint32_t* out, const ModP& m, hipStream_t st);
 // *pos += by (one thread)
 void launch_advance_pos(uint32_t* pos, uint32_t by, hipStream_t st);
 

@@ -402,6 +402,7 @@ void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64
 
 constexpr int kGramRows = 64;
 constexpr unsigned kGramGrid = 1024;
+constexpr unsigned kGramNoallocGrid = 256;  // workgroups of the allocation-free variant
 constexpr uint64_t kGramRowsPerBlock = 256; // least rows per partial workgroup
 
 size_t gram_scratch_bytes(uint32_t b) {
@@ -541,6 +542,47 @@ void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b
                  uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
                  hipStream_t st, const uint32_t* pos, uint32_t pos_off) {
     gram_launch(x, y, rows, b, mode, out0, out1, scratch, kGramGrid, m, st, pos, pos_off);
+}
+
+// ---------------------------------------------------------------------------
+// Gram projection without any allocation: the partials live in a fixed
+// device array (one copy per GPU, held until the process ends) instead of an
+// operator's scratch.  Every partial a call reads is written by the same
+// call, so nothing needs clearing; calls that overlap on one GPU share the
+// array and race.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) uint64_t scratch[(size_t)kGramNoallocGrid * 2 * 256 * 2];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mtx;
+uint64_t* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+size_t gram_noalloc_scratch_bytes() { return sizeof(scratch); }
+
+void gram_noalloc_setup(int device) {
+    require(device >= 0 && device < kMaxDevices, "device index outside the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mtx);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;  // the array's address on the current device
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        throw Error(PGW_ERR_HIP, std::string("hipGetSymbolAddress of the Gram scratch failed: ") +
+                                     hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<uint64_t*>(ptr);
+}
+
+void launch_gram_noalloc(int device, const uint32_t* x, const uint32_t* y, uint64_t rows,
+                         uint32_t b, uint32_t* out, const ModP& m, hipStream_t st) {
+    uint64_t* part = nullptr;
+    if (device >= 0 && device < kMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mtx);
+        part = g_scratch_ptr[device];
+    }
+    require(part != nullptr, "the Gram scratch of this device was not set up (pgw_ctx_create)");
+    gram_launch(x, y, rows, b, 0, out, nullptr, part, kGramNoallocGrid, m, st, nullptr, 0);
 }
 
 // ===========================================================================
