@@ -1,17 +1,24 @@
 """Block-apply throughput of one sector operator on one MI355X.
 
-Workload: the single mid-size sector (n=20, k=5, M=200, M'=220,
-N = C(20,5)*200 = 3,100,800 rows, p = 2^31-1, 16-wide uint32 block, V from
-seed 2, two rounds of row mixing).  One step = one block-apply X -> S X with
-S = Dc F^T P^T Dr P F Dc, F = E o D, on the int8-limb MFMA path; inputs are
-resident in HBM before the timed region.  The plain u64 path is timed beside
-it on the same input, and the two outputs must be bit-identical before any
-number is printed.
+Workload (default `large`): the HBM-scale sector n=24, k=6, M=400, M'=440,
+N = C(24,6)*400 = 53,838,400 rows, p = 2^31-1, 16-wide uint32 block (3.4 GB),
+tables from seeds 4/5, V from seed 4, two rounds of row mixing.  One step =
+one block-apply X -> S X with S = Dc F^T P^T Dr P F Dc, F = E o D, on the
+int8-limb MFMA path; inputs are resident in HBM before the timed region.  The
+plain u64 path is timed beside it on the same input, and the two outputs must
+be bit-identical before any number is printed.
 
-Other workloads (`--workload`): `large` (n=24, k=6, M=400, M'=440,
-N = 53,838,400, tables from seed 4) the same way; `batch` (64 uneven sectors,
-batched against one at a time); `serve` (a Poisson stream of sector jobs
-through the serving loop, p50/p99 latency).
+Other workloads (`--workload`): `sector` (the mid sector n=20, k=5, M=200,
+M'=220, N = 3,100,800) the same way; `batch` (64 uneven sectors, batched
+against one at a time); `serve` (a Poisson stream of sector jobs through the
+serving loop, p50/p99 latency).
+
+The JSON line carries `roofline` (the MFMA sweep against dense int8 peak;
+`traffic` from the FETCH_SIZE/WRITE_SIZE counters committed in
+profiles/traffic_<workload>.txt when they exist), `hbm_stages` (the
+HBM-bound stages of the apply against 8 TB/s, timed live with HIP events) and
+`cpu_baseline` (the oracle's whole apply chain, sampled, at 1 and at all
+host threads).
 
 Prints one JSON line (rank 0).  `--gpus` above 1 exits: this build runs on
 one GPU.
@@ -48,36 +55,88 @@ def macs_per_sweep(n, k, M, Mp, b):
 
 
 def cpu_baseline(spec, T, E, dr, dc, pairs, v, b, budget_s=20.0):
-    """Oracle D-sweep over a bounded prefix of target rows, extrapolated to a
-    whole block-apply (4 sweeps of equal work)."""
+    """The oracle's whole apply chain (the D, E, E^T and D^T sweeps and the
+    elementwise Dc / P / Dr / P^T / Dc stages), each timed on a bounded sample
+    of its output rows and scaled to all rows, at one host thread and at all
+    (at most 16)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_ctypes as oc
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    L = oc.lib()
+    nthreads = min(16, os.cpu_count() or 1)
     o = oc.synthetic_op(spec.n, spec.k, spec.m_src, spec.m_tgt, spec.prime, T, E, dr, dc, 2,
                         pairs.reshape(-1))
-    R = oc.lib().orc_sweep_rows(o.h, 0)
-    out = np.zeros(R * b, np.uint32)
+    N, R = spec.cols, comb(spec.n, spec.k - 1) * spec.m_tgt
     x = np.ascontiguousarray(v)
-    rows = max(64, R // 2000)
-    elapsed, done = 0.0, 0
-    while elapsed < budget_s and done < R:
-        r1 = min(R, done + rows)
-        t0 = time.perf_counter()
-        oc.lib().orc_sweep_partial(o.h, 0, x, out, b, done, r1)
-        elapsed += time.perf_counter() - t0
-        done = r1
-        rows *= 2
-    sec_per_apply = 4.0 * elapsed * (R / done)
+    y = np.zeros(R * b, np.uint32)          # R-space input of the E and D^T sweeps
+    outs = {0: np.zeros(R * b, np.uint32), 1: np.zeros(N * b, np.uint32)}
+    scratch = x.copy()
+
+    def sampled(threads, budget):
+        L.orc_set_threads(threads)
+        total, notes = 0.0, []
+        # growing chunks of output rows; the last (largest) chunk's time per
+        # row, which keeps every thread busy, scales to the whole sweep
+        for which in range(4):
+            rows = L.orc_sweep_rows(o.h, which)
+            xin, out = (x, outs[0]) if which in (0, 2) else (y, outs[1])
+            step, done, el, per_row = max(256 * threads, rows // 4000), 0, 0.0, 0.0
+            while el < budget / 5 and done < rows:
+                r1 = min(rows, done + step)
+                t0 = time.perf_counter()
+                L.orc_sweep_partial(o.h, which, xin, out, b, done, r1)
+                dt = time.perf_counter() - t0
+                el, per_row = el + dt, dt / (r1 - done)
+                done, step = r1, step * 2
+            total += per_row * rows
+            notes.append(f"{done}/{rows}")
+        rows_el, el = 4096, 0.0
+        while True:
+            t0 = time.perf_counter()
+            L.orc_elementwise_sample(o.h, scratch, b, rows_el)
+            el = time.perf_counter() - t0
+            if el >= budget / 10 or rows_el >= N:
+                break
+            rows_el = min(N, rows_el * 2)
+        total += el * N / rows_el
+        notes.append(f"elementwise {rows_el}/{N}")
+        return total, notes
+
+    t_single, n1 = sampled(1, budget_s * 0.4)
+    t_multi, nm = sampled(nthreads, budget_s * 0.6)
     return {
-        "value": 1.0 / sec_per_apply,
+        "value": 1.0 / t_multi,
         "unit": "block-applies/s",
-        "cores": threads,
+        "cores": nthreads,
         "kind": "port",
-        "sample": f"oracle D sweep over {done} of {R} target rows ({elapsed:.1f} s, {threads} "
-                  f"OpenMP threads), x{R / done:.1f} rows and x4 sweeps per block-apply",
+        "single_thread_value": 1.0 / t_single,
+        "sample": f"oracle apply chain, each stage timed on sampled output rows and scaled: "
+                  f"D, E, E^T, D^T sweeps over {', '.join(nm[:4])} rows and the elementwise "
+                  f"stages over {nm[4].split()[1]} rows at {nthreads} OpenMP threads "
+                  f"({t_multi:.0f} s per block-apply); at 1 thread {', '.join(n1[:4])} rows "
+                  f"({t_single:.0f} s per block-apply)",
     }
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary profiles/traffic_<workload>.txt (`key value` lines: FETCH_SIZE and
+    WRITE_SIZE KiB per dispatch of the forward and transpose sweeps and the
+    copy-kernel calibration factors), or None."""
+    path = ROOT / "profiles" / f"traffic_{workload}.txt"
+    if not path.exists():
+        return None
+    kv = {}
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#"):
+            kv[parts[0]] = float(parts[1])
+    need = ("fetch_kib_fwd", "write_kib_fwd", "fetch_kib_trn", "write_kib_trn", "cal_fetch", "cal_write")
+    if not all(k in kv for k in need):
+        return None
+    per = [(kv[f"fetch_kib_{d}"] * kv["cal_fetch"] + kv[f"write_kib_{d}"] * kv["cal_write"]) * 1024
+           for d in ("fwd", "trn")]
+    return sum(per) / 2
 
 
 def sector_macs(spec, b):
@@ -347,11 +406,32 @@ def main():
     ops_per_sweep = 2.0 * macs_per_sweep(n, k, M, Mp, b) * nl * nl  # int8 ops, algorithmic
     achieved = ops_per_sweep / (sweep_ms * 1e-3) / 1e12
     peak = 5000.0  # TOP/s dense int8, MI355X
+    traffic = traffic_from_profiles(args.workload)
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
+                "frac": round(achieved / peak, 4),
+                "traffic": None if traffic is None else float(f"{traffic:.4g}"),
                 "kernel": "k_dense_mfma", "sweep_ms": round(sweep_ms, 4),
                 "sweeps_per_apply": sweeps, "tile_configs": tile_choice,
-                "tile_ms": [round(t, 4) for t in tile_ms]}
+                "tile_ms": [round(t, 4) for t in tile_ms],
+                "ops_per_launch": ops_per_sweep}
+
+    # the HBM-bound stages of the apply, HIP events around every launch
+    stages = op.bench_stages(a.data_ptr(), bb.data_ptr(), 3)
+    hbm_peak = 8000.0  # GB/s, MI355X HBM3E
+    hbm = {}
+    for name in ("digit_split", "row_mixing"):
+        if name in stages:
+            ms, by, nlaunch = stages[name]
+            gbs = by / (ms * 1e-3) / 1e9
+            hbm[name] = {"ms_per_apply": round(ms, 4), "launches": nlaunch, "bytes": by,
+                         "GB/s": round(gbs, 1), "frac": round(gbs / hbm_peak, 4)}
+    if hbm:
+        ms_all = sum(h["ms_per_apply"] for h in hbm.values())
+        by_all = sum(h["bytes"] for h in hbm.values())
+        hbm["elementwise_total"] = {"ms_per_apply": round(ms_all, 4), "bytes": by_all,
+                                    "GB/s": round(by_all / (ms_all * 1e-3) / 1e9, 1),
+                                    "frac": round(by_all / (ms_all * 1e-3) / 1e9 / hbm_peak, 4),
+                                    "share_of_apply": round(ms_all / ms_per_step, 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -382,6 +462,7 @@ def main():
         "wall_s_timed": round(wall, 3),
         "device": dev_name,
         "roofline": roofline,
+        "hbm_stages": hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

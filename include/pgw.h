@@ -228,6 +228,25 @@ int pgw_operator_autotune(pgw_op* op, uint32_t reps, uint32_t* h_choice, double*
 int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters,
                     double* total_ms, double* sweep_avg_ms, uint32_t* sweeps_per_apply);
 
+/* Stages of a block-apply for pgw_bench_stages. */
+enum pgw_stage {
+    PGW_STAGE_SWEEP = 0,      /* table-contraction sweeps (MFMA or plain) */
+    PGW_STAGE_DIGITS = 1,     /* u32 block -> int8 digit planes (MFMA path) */
+    PGW_STAGE_ROWMIX = 2,     /* P^T Dr P on the target block */
+    PGW_STAGE_SCALE = 3,      /* Dc row scaling (plain path) */
+    PGW_STAGE_SPARSE = 4      /* sparse maps of the cyclic operator */
+};
+#define PGW_STAGES 5
+/* Per-stage timing of the apply on the operator's own stream: `iters`
+ * block-applies (ping-pong between d_a and d_b), HIP events around every
+ * launch.  h_ms[s] = mean milliseconds per apply in stage s, h_bytes[s] = the
+ * stage's algorithmic HBM bytes per apply (unique input + output bytes),
+ * h_launches[s] = its launches per apply; arrays of PGW_STAGES entries.  No
+ * counterpart in the reference (it times only the whole loop,
+ * cuda/cuprym_cyclic.cu:2420-2461). */
+int pgw_bench_stages(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, double* h_ms,
+                     double* h_bytes, uint32_t* h_launches);
+
 /* ------------------------------------------------------------------------
  * Batches: many sectors' sequence loops advanced together.
  *

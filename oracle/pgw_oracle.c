@@ -15,6 +15,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef unsigned __int128 u128;
 
@@ -923,6 +926,39 @@ void orc_sweep_partial(orc_op* op, int which, const uint32_t* in, uint32_t* out,
         case 3: phi_transpose_range(&op->d_op, in, out, b, op->p, row0, row1); break;  /* D^T */
         default: break;
     }
+}
+
+void orc_elementwise_sample(orc_op* op, uint32_t* block, size_t b, size_t rows) {
+    const uint32_t p = op->p;
+    const size_t r = rows < op->cols ? rows : op->cols;
+    const size_t pc = op->rows / 2, qmax = r / 2 < pc ? r / 2 : pc;
+    scale_rows(block, op->dc, block, r, b, p);                      /* Dc on the input */
+    for (int transpose = 0; transpose < 2; ++transpose) {           /* P, then P^T */
+        for (size_t t = 0; t < op->rounds; ++t) {
+            const size_t round = transpose ? op->rounds - 1 - t : t;
+            const uint32_t* pr = op->pairs + round * pc * 4;
+            for (size_t q = 0; q < qmax; ++q) {
+                const size_t f = pr[4 * q], s2 = pr[4 * q + 1];
+                const uint64_t a = pr[4 * q + 2], bb = pr[4 * q + 3];
+                const uint64_t opab = (1 + (a * bb) % p) % p;
+                for (size_t c = 0; c < b; ++c) {
+                    const uint64_t x = block[f * b + c], y = block[s2 * b + c];
+                    block[f * b + c] = (uint32_t)((transpose ? x + bb * y : x + a * y) % p);
+                    block[s2 * b + c] = (uint32_t)((transpose ? a * x + opab * y : bb * x + opab * y) % p);
+                }
+            }
+        }
+        if (!transpose) scale_rows(block, op->dr, block, r < op->rows ? r : op->rows, b, p);  /* Dr */
+    }
+    scale_rows(block, op->dc, block, r, b, p);                      /* Dc on the output */
+}
+
+void orc_set_threads(int threads) {
+#ifdef _OPENMP
+    omp_set_num_threads(threads > 0 ? threads : 1);
+#else
+    (void)threads;
+#endif
 }
 
 size_t orc_sweep_rows(const orc_op* op, int which) {

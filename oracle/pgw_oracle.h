@@ -118,6 +118,13 @@ void orc_apply_square(orc_op* op, const uint32_t* in, uint32_t* out, size_t b);
  * must hold the whole sweep output. */
 void orc_sweep_partial(orc_op* op, int which, const uint32_t* in, uint32_t* out, size_t b,
                        size_t row0, size_t row1);
+/* Timing sample of the elementwise stages of one apply (cpu_baseline only):
+ * Dc scaling of the first `rows` rows, the row-mixing rounds (then their
+ * transposes) on the pairs in the first rows/2 slots of each round, Dr and Dc
+ * scaling.  `block` holds the operator's full row count x b and is scratch. */
+void orc_elementwise_sample(orc_op* op, uint32_t* block, size_t b, size_t rows);
+/* OpenMP threads of the oracle's parallel loops. */
+void orc_set_threads(int threads);
 /* Output rows of a synthetic sweep (R for D and E^T, N for E and D^T). */
 size_t orc_sweep_rows(const orc_op* op, int which);
 

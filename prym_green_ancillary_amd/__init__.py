@@ -104,7 +104,8 @@ EXPORTED = [
     "pgw_server_submit", "pgw_server_wait", "pgw_server_result", "pgw_server_stats",
     "pgw_server_destroy", "pgw_job_footprint", "pgw_rank_create", "pgw_rank_step",
     "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
-    "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_gram_upper_noalloc",
+    "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_bench_stages",
+    "pgw_gram_upper_noalloc",
 ]
 
 
@@ -153,6 +154,7 @@ def lib():
         "pgw_bench_apply": (I, [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u32)]),
         "pgw_operator_autotune": (I, [vp, u32, vp, vp]),
+        "pgw_bench_stages": (I, [vp, vp, vp, u32, vp, vp, vp]),
         "pgw_batch_create": (I, [vp, u32, u32, ctypes.POINTER(vp)]),
         "pgw_batch_add": (I, [vp, vp, vp, u64, ctypes.POINTER(u64)]),
         "pgw_batch_step": (I, [vp, u32, ctypes.POINTER(u64)]),
@@ -311,8 +313,10 @@ class Context:
         _check(lib().pgw_gram_upper_noalloc(self.h, prime, block, rows, d_x, d_y, d_out, stream))
 
     def close(self):
+        """Free the context; refused (PgwError) while operators, batches or
+        servers created on it are still open."""
         if getattr(self, "h", None):
-            lib().pgw_ctx_destroy(self.h)
+            _check(lib().pgw_ctx_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -432,6 +436,16 @@ class Operator:
         _check(lib().pgw_bench_apply(self.h, d_a, d_b, iters, ctypes.byref(total),
                                      ctypes.byref(sweep), ctypes.byref(count)))
         return total.value, sweep.value, count.value
+
+    STAGES = ("sweep", "digit_split", "row_mixing", "row_scaling", "sparse_maps")
+
+    def bench_stages(self, d_a: int, d_b: int, iters: int = 3):
+        """Per-stage time of one apply (pgw_bench_stages): {stage: (ms, bytes, launches)}."""
+        ms = np.zeros(len(self.STAGES), np.float64)
+        by = np.zeros(len(self.STAGES), np.float64)
+        n = np.zeros(len(self.STAGES), np.uint32)
+        _check(lib().pgw_bench_stages(self.h, d_a, d_b, iters, _ptr(ms), _ptr(by), _ptr(n)))
+        return {s: (float(ms[q]), float(by[q]), int(n[q])) for q, s in enumerate(self.STAGES) if n[q]}
 
     def autotune(self, reps: int = 3):
         """Pick the fastest MFMA tile configuration per sweep; returns (choices, ms)."""

@@ -21,6 +21,7 @@ SellDevice upload_sell(DeviceArena& A, const CsrHost& csr) {
     d.rows = h.rows;
     d.cols = h.cols;
     d.nnz = h.nnz;
+    d.padded = h.slice.back();
     d.slice = A.upload(h.slice);
     d.ent = A.upload(h.ent);
     return d;
@@ -54,11 +55,14 @@ void check_mfma_bound(const DenseSweep& s, uint32_t nl) {
             PGW_ERR_UNSUPPORTED);
 }
 
-struct SweepTimer {
+// Events around one stage launch while op->profile is set (benchmarks only).
+struct StageTimer {
     pgw_op* op;
     hipStream_t st;
+    int stage;
+    double bytes;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    SweepTimer(pgw_op* o, hipStream_t s) : op(o), st(s) {
+    StageTimer(pgw_op* o, int stg, double by, hipStream_t s) : op(o), st(s), stage(stg), bytes(by) {
         if (op->profile) {
             PGW_HIP(hipEventCreate(&e0));
             PGW_HIP(hipEventCreate(&e1));
@@ -68,17 +72,50 @@ struct SweepTimer {
     void done() {
         if (op->profile) {
             PGW_HIP(hipEventRecord(e1, st));
-            op->sweep_events.emplace_back(e0, e1);
+            op->stage_events.push_back({stage, bytes, e0, e1});
         }
     }
 };
+
+// algorithmic bytes of the elementwise stages (each stage's unique input and
+// output bytes, the quantities an HBM roofline prices)
+double limb_split_bytes(uint64_t v, uint32_t k_in, uint32_t b, uint32_t nl, bool scaled) {
+    const double elems = (double)v * k_in;
+    return elems * b * 4 + (scaled ? elems * 4 : 0) + 2.0 * nl * v * b * limb_stride(k_in);
+}
+double rowmix_bytes(const pgw_op* op) {
+    const double rb = (double)op->rows * op->b * 4;
+    if (op->rounds == 0 || op->pair_count == 0) return 2 * rb + op->rows * 4.0;
+    // 2R-1 launches, each reading and writing the paired rows and its pair records
+    return (2.0 * op->rounds - 1) * (2 * rb + 16.0 * op->pair_count) + op->rows * 4.0;
+}
+double sell_bytes(const SellDevice& M, uint32_t b) {
+    const uint64_t slices = (M.rows + kSellHeight - 1) / kSellHeight;
+    return (double)M.padded * 8 + (slices + 1) * 8.0 + (double)M.cols * b * 4 + (double)M.rows * b * 4;
+}
+
+void timed_limb_split(pgw_op* op, const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
+                      uint32_t k_in, hipStream_t st) {
+    StageTimer t(op, 1, limb_split_bytes(v, k_in, op->b, op->nl, scale != nullptr), st);
+    launch_limb_split(in, scale, lp, v, k_in, op->b, op->nl, op->m, st);
+    t.done();
+}
+
+void timed_sell(pgw_op* op, const SellDevice& M, const uint32_t* in, uint32_t* out, hipStream_t st) {
+    StageTimer t(op, 4, sell_bytes(M, op->b), st);
+    launch_sell_spmm(M, in, out, op->b, op->m, st);
+    t.done();
+}
 
 // in: u32 block (plain) ; the MFMA path consumes the digit planes prepared by
 // the caller in op->d_lp_src / d_lp_mid
 void run_sweep(pgw_op* op, int which, const uint32_t* in_u32, const int8_t* in_lp, uint32_t* out,
                const uint32_t* scale, hipStream_t st) {
     const DenseSweep s = sweep_desc(op, which);
-    SweepTimer t(op, st);
+    const double in_bytes = op->method == PGW_METHOD_MFMA
+                                ? 2.0 * op->nl * s.v_in * op->b * limb_stride(s.k_in)
+                                : (double)s.v_in * s.k_in * op->b * 4;
+    StageTimer t(op, 0, in_bytes + (double)s.u_out * s.r_out * op->b * 4, st);
     if (op->method == PGW_METHOD_MFMA)
         launch_dense_mfma(s, op->mt[which], in_lp, out, scale, op->b, op->nl, op->m, st,
                           op->mfma_cfg[which]);
@@ -93,19 +130,23 @@ void forward_into_target(pgw_op* op, const uint32_t* x, hipStream_t st) {
     const ModP& m = op->m;
     if (op->kind == PGW_KIND_SYNTHETIC) {
         if (op->method == PGW_METHOD_MFMA) {
-            launch_limb_split(x, op->d_dc, op->d_lp_src, op->u_src, op->M, op->b, op->nl, m, st);
+            timed_limb_split(op, x, op->d_dc, op->d_lp_src, op->u_src, op->M, st);
             run_sweep(op, 0, nullptr, op->d_lp_src, op->d_mid, nullptr, st);
-            launch_limb_split(op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, op->b, op->nl, m, st);
+            timed_limb_split(op, op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, st);
             run_sweep(op, 1, nullptr, op->d_lp_mid, op->d_target, nullptr, st);
         } else {
-            launch_scale_rows(x, op->d_dc, op->d_scaled, op->cols, op->b, m, st);
+            {
+                StageTimer t(op, 3, 2.0 * op->cols * op->b * 4 + op->cols * 4.0, st);
+                launch_scale_rows(x, op->d_dc, op->d_scaled, op->cols, op->b, m, st);
+                t.done();
+            }
             run_sweep(op, 0, op->d_scaled, nullptr, op->d_mid, nullptr, st);
             run_sweep(op, 1, op->d_mid, nullptr, op->d_target, nullptr, st);
         }
     } else {
         // F (Dc x) of the cyclic operator: the first two sparse maps
-        launch_sell_spmm(op->m1, x, op->d_y, op->b, m, st);
-        launch_sell_spmm(op->m2, op->d_y, op->d_target, op->b, m, st);
+        timed_sell(op, op->m1, x, op->d_y, st);
+        timed_sell(op, op->m2, op->d_y, op->d_target, st);
     }
 }
 
@@ -113,9 +154,9 @@ void forward_into_target(pgw_op* op, const uint32_t* x, hipStream_t st) {
 void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
     const ModP& m = op->m;
     if (op->method == PGW_METHOD_MFMA) {
-        launch_limb_split(op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, op->b, op->nl, m, st);
+        timed_limb_split(op, op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, st);
         run_sweep(op, 2, nullptr, op->d_lp_src, op->d_mid, nullptr, st);
-        launch_limb_split(op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, op->b, op->nl, m, st);
+        timed_limb_split(op, op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, st);
         run_sweep(op, 3, nullptr, op->d_lp_mid, out, op->d_dc, st);
     } else {
         run_sweep(op, 2, op->d_target, nullptr, op->d_mid, nullptr, st);
@@ -129,17 +170,20 @@ void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
 void pgw::apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st) {
     if (op->kind != PGW_KIND_SYNTHETIC) {
         // S = M1^T M2^T M3 M2 M1: Dc, the elimination and P^T Dr P are folded in
-        const ModP& m = op->m;
-        launch_sell_spmm(op->m1, in, op->d_y, op->b, m, st);
-        launch_sell_spmm(op->m2, op->d_y, op->d_target, op->b, m, st);
-        launch_sell_spmm(op->m3, op->d_target, op->d_t2, op->b, m, st);
-        launch_sell_spmm(op->m2t, op->d_t2, op->d_y, op->b, m, st);
-        launch_sell_spmm(op->m1t, op->d_y, out, op->b, m, st);
+        timed_sell(op, op->m1, in, op->d_y, st);
+        timed_sell(op, op->m2, op->d_y, op->d_target, st);
+        timed_sell(op, op->m3, op->d_target, op->d_t2, st);
+        timed_sell(op, op->m2t, op->d_t2, op->d_y, st);
+        timed_sell(op, op->m1t, op->d_y, out, st);
         return;
     }
     forward_into_target(op, in, st);
-    launch_target_precond(op->d_target, op->d_pairs, op->pair_count, op->rounds, op->d_dr,
-                          op->unpaired, op->rows, op->b, op->m, st);
+    {
+        StageTimer t(op, 2, rowmix_bytes(op), st);
+        launch_target_precond(op->d_target, op->d_pairs, op->pair_count, op->rounds, op->d_dr,
+                              op->unpaired, op->rows, op->b, op->m, st);
+        t.done();
+    }
     backward_from_target(op, out, st);
 }
 
@@ -234,6 +278,9 @@ int pgw_ctx_create(int device, pgw_ctx** out) {
 int pgw_ctx_destroy(pgw_ctx* ctx) {
     return guarded([&] {
         if (!ctx) return;
+        const uint64_t live = ctx->live.load();
+        require(live == 0, "context still owns " + std::to_string(live) +
+                               " operators or batches: destroy them first");
         PGW_HIP(hipSetDevice(ctx->device));
         PGW_HIP(hipStreamSynchronize(ctx->stream));
         PGW_HIP(hipStreamDestroy(ctx->stream));
@@ -451,6 +498,7 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
         }
         op->d_target = A.alloc<uint32_t>(op->rows * b);
         op->d_gram = A.alloc<uint64_t>(gram_scratch_bytes(b) / sizeof(uint64_t));
+        ctx->live.fetch_add(1);
         *out = op;
     });
     if (rc != PGW_OK && op) {
@@ -467,12 +515,13 @@ int pgw_operator_destroy(pgw_op* op) {
         {
             std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
             PGW_HIP(hipStreamSynchronize(op->ctx->stream));
-            for (auto& e : op->sweep_events) {
-                (void)hipEventDestroy(e.first);
-                (void)hipEventDestroy(e.second);
+            for (auto& e : op->stage_events) {
+                (void)hipEventDestroy(e.e0);
+                (void)hipEventDestroy(e.e1);
             }
         }
         op->arena.release();
+        op->ctx->live.fetch_sub(1);
         delete op;
     });
 }
@@ -695,17 +744,50 @@ int pgw_operator_autotune(pgw_op* op, uint32_t reps, uint32_t* h_choice, double*
     });
 }
 
+namespace {
+
+// one profiled apply: per stage, summed milliseconds, bytes and launches
+void profile_one_apply(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st, double* ms,
+                       double* bytes, uint32_t* launches) {
+    for (auto& e : op->stage_events) {
+        (void)hipEventDestroy(e.e0);
+        (void)hipEventDestroy(e.e1);
+    }
+    op->stage_events.clear();
+    op->profile = true;
+    try {
+        apply_square_impl(op, in, out, st);
+    } catch (...) {
+        op->profile = false;
+        throw;
+    }
+    op->profile = false;
+    PGW_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < PGW_STAGES; ++q) {
+        ms[q] = 0;
+        bytes[q] = 0;
+        launches[q] = 0;
+    }
+    for (auto& e : op->stage_events) {
+        float t = 0;
+        PGW_HIP(hipEventElapsedTime(&t, e.e0, e.e1));
+        ms[e.stage] += t;
+        bytes[e.stage] += e.bytes;
+        launches[e.stage] += 1;
+        (void)hipEventDestroy(e.e0);
+        (void)hipEventDestroy(e.e1);
+    }
+    op->stage_events.clear();
+}
+
+}  // namespace
+
 int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, double* total_ms,
                     double* sweep_avg_ms, uint32_t* sweeps_per_apply) {
     return guarded([&] {
         require(op && d_a && d_b && iters >= 1, "null operator/block or zero iterations");
         PGW_HIP(hipSetDevice(op->ctx->device));
         hipStream_t st = op->ctx->stream;
-        for (auto& e : op->sweep_events) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
-        op->sweep_events.clear();
         hipEvent_t t0, t1;
         PGW_HIP(hipEventCreate(&t0));
         PGW_HIP(hipEventCreate(&t1));
@@ -718,26 +800,36 @@ int pgw_bench_apply(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, do
         PGW_HIP(hipEventSynchronize(t1));
         float ms = 0;
         PGW_HIP(hipEventElapsedTime(&ms, t0, t1));
-        if (total_ms) *total_ms = ms;
-        // per-launch sweep durations: one more apply with events around each sweep
-        op->profile = op->kind == PGW_KIND_SYNTHETIC;
-        apply_square_impl(op, (iters % 2) ? d_b : d_a, (iters % 2) ? d_a : d_b, st);
-        op->profile = false;
-        PGW_HIP(hipStreamSynchronize(st));
-        double sum = 0;
-        for (auto& e : op->sweep_events) {
-            float s = 0;
-            PGW_HIP(hipEventElapsedTime(&s, e.first, e.second));
-            sum += s;
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
-        const size_t count = op->sweep_events.size();
-        op->sweep_events.clear();
-        if (sweep_avg_ms) *sweep_avg_ms = count ? sum / (double)count : 0.0;
-        if (sweeps_per_apply) *sweeps_per_apply = (uint32_t)count;
         (void)hipEventDestroy(t0);
         (void)hipEventDestroy(t1);
+        if (total_ms) *total_ms = ms;
+        // per-launch sweep durations: one more apply with events around each stage
+        double sms[PGW_STAGES], sbytes[PGW_STAGES];
+        uint32_t sl[PGW_STAGES];
+        profile_one_apply(op, (iters % 2) ? d_b : d_a, (iters % 2) ? d_a : d_b, st, sms, sbytes, sl);
+        if (sweep_avg_ms) *sweep_avg_ms = sl[0] ? sms[0] / sl[0] : 0.0;
+        if (sweeps_per_apply) *sweeps_per_apply = sl[0];
+    });
+}
+
+int pgw_bench_stages(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, double* h_ms,
+                     double* h_bytes, uint32_t* h_launches) {
+    return guarded([&] {
+        require(op && d_a && d_b && iters >= 1 && h_ms && h_bytes && h_launches,
+                "null operator, block or output, or zero iterations");
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        hipStream_t st = op->ctx->stream;
+        for (int q = 0; q < PGW_STAGES; ++q) h_ms[q] = h_bytes[q] = 0, h_launches[q] = 0;
+        double ms[PGW_STAGES], bytes[PGW_STAGES];
+        uint32_t launches[PGW_STAGES];
+        for (uint32_t it = 0; it < iters; ++it) {
+            profile_one_apply(op, (it % 2) ? d_b : d_a, (it % 2) ? d_a : d_b, st, ms, bytes, launches);
+            for (int q = 0; q < PGW_STAGES; ++q) {
+                h_ms[q] += ms[q] / iters;
+                h_bytes[q] = bytes[q];
+                h_launches[q] = launches[q];
+            }
+        }
     });
 }
 

@@ -216,6 +216,7 @@ int pgw_batch_create(pgw_ctx* ctx, uint32_t nstreams, uint32_t flags, pgw_batch*
         }
         for (auto& u : b->upload) PGW_HIP(hipStreamCreateWithFlags(&u, hipStreamNonBlocking));
         PGW_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        ctx->live.fetch_add(1);
         *out = b;
     });
     if (rc != PGW_OK && b) {
@@ -368,6 +369,7 @@ int pgw_batch_destroy(pgw_batch* b) {
             for (auto u : b->upload) (void)hipStreamDestroy(u);
             (void)hipEventDestroy(b->fork);
         }
+        b->ctx->live.fetch_sub(1);
         delete b;
     });
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <stdexcept>
@@ -172,6 +173,9 @@ struct pgw_ctx {
     // create/destroy, batch add/result/cancel): such calls during a capture
     // invalidate it.
     std::mutex capture_mtx;
+    // operators and batches created on this context and not yet destroyed:
+    // the context is not freed while any of them could still use it
+    std::atomic<uint64_t> live{0};
 };
 
 namespace pgw {

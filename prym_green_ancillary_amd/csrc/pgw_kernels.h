@@ -40,7 +40,7 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
                         uint32_t b, const ModP& m, hipStream_t st);
 // Sliced-ELL sparse map on the device (layout: SellHost, pgw_internal.h).
 struct SellDevice {
-    uint64_t rows = 0, cols = 0, nnz = 0;
+    uint64_t rows = 0, cols = 0, nnz = 0, padded = 0;  // padded: stored entries
     const uint64_t* slice = nullptr;
     const uint32_t* ent = nullptr;
 };

@@ -177,70 +177,109 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 //
 // out (rows x B) = M in (cols x B) mod p for a sliced-ELL map: the entries of
 // slice s (kSellHeight consecutive rows) are interleaved row by row, so that
-// the 64 lanes of a wavefront read one 512-byte run per step.  CW columns per
-// thread (B / CW threads per row): a 16-byte load gathers four of them.
+// the lanes of a wavefront that work on consecutive rows read consecutive
+// entries.  A row is handled by B / CW column groups (a 16-byte load gathers
+// four columns) times LANES lanes that split its entries (lane l takes entries
+// l, l + LANES, ...) and add their sums with cross-lane shuffles: short rows of
+// small sectors get enough threads to hide the entry -> gather latency chain.
 // SMALLP (p < 2^16): products < 2^32, summed in u64; otherwise products
 // < 2^62 summed as 64-bit low/high word sums.  Exact for rows of any length
 // below 2^32.
-template <int B, int CW, bool SMALLP>
+template <int B, int CW, int LANES, bool SMALLP>
 __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ slice,
                                                    const uint2* __restrict__ ent, uint64_t rows,
                                                    const uint32_t* __restrict__ in,
                                                    uint32_t* __restrict__ out, ModP m) {
     constexpr int G = B / CW;
+    constexpr int T = G * LANES;  // threads per row; LANES > 1 only when T divides 64
     const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t row = tid / G;
-    if (row >= rows) return;
-    const uint32_t g = (uint32_t)(tid % G);
-    const uint64_t sl = row / kSellHeight;
-    const uint64_t b0 = slice[sl], width = (slice[sl + 1] - b0) / kSellHeight;
-    const uint2* e = ent + b0 + row % kSellHeight;
+    const uint64_t row = tid / T;
+    const uint32_t g = (uint32_t)((tid % T) / LANES), lane = (uint32_t)(tid % LANES);
+    const bool valid = row < rows;
     uint64_t lo[CW], hi[CW];
 #pragma unroll
     for (int q = 0; q < CW; ++q) lo[q] = hi[q] = 0;
-    for (uint64_t k = 0; k < width; ++k) {
-        const uint2 c = e[k * kSellHeight];
-        const uint32_t* xr = in + (uint64_t)c.x * B + g * CW;
-        uint32_t xv[CW];
-        if constexpr (CW == 4) {
-            const uint4 v = *reinterpret_cast<const uint4*>(xr);
-            xv[0] = v.x; xv[1] = v.y; xv[2] = v.z; xv[3] = v.w;
-        } else {
-#pragma unroll
-            for (int q = 0; q < CW; ++q) xv[q] = xr[q];
-        }
-#pragma unroll
-        for (int q = 0; q < CW; ++q) {
-            const uint64_t prod = (uint64_t)c.y * xv[q];
-            if (SMALLP) {
-                lo[q] += prod;
+    if (valid) {
+        const uint64_t sl = row / kSellHeight;
+        const uint64_t b0 = slice[sl], width = (slice[sl + 1] - b0) / kSellHeight;
+        const uint2* e = ent + b0 + row % kSellHeight;
+        for (uint64_t k = lane; k < width; k += LANES) {
+            const uint2 c = e[k * kSellHeight];
+            const uint32_t* xr = in + (uint64_t)c.x * B + g * CW;
+            uint32_t xv[CW];
+            if constexpr (CW == 4) {
+                const uint4 v = *reinterpret_cast<const uint4*>(xr);
+                xv[0] = v.x; xv[1] = v.y; xv[2] = v.z; xv[3] = v.w;
             } else {
-                lo[q] += (uint32_t)prod;
-                hi[q] += prod >> 32;
+#pragma unroll
+                for (int q = 0; q < CW; ++q) xv[q] = xr[q];
+            }
+#pragma unroll
+            for (int q = 0; q < CW; ++q) {
+                const uint64_t prod = (uint64_t)c.y * xv[q];
+                if (SMALLP) {
+                    lo[q] += prod;
+                } else {
+                    lo[q] += (uint32_t)prod;
+                    hi[q] += prod >> 32;
+                }
             }
         }
     }
+    if constexpr (LANES > 1) {
+        // the lanes of one (row, group) are adjacent: xor offsets below LANES stay inside it
 #pragma unroll
-    for (int q = 0; q < CW; ++q)
-        out[row * B + g * CW + q] = SMALLP ? mod_u64(lo[q], m) : mod_split(hi[q], lo[q], m);
+        for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int q = 0; q < CW; ++q) {
+                lo[q] += __shfl_xor(lo[q], off, 64);
+                if (!SMALLP) hi[q] += __shfl_xor(hi[q], off, 64);
+            }
+    }
+    if (valid && lane == 0) {
+#pragma unroll
+        for (int q = 0; q < CW; ++q)
+            out[row * B + g * CW + q] = SMALLP ? mod_u64(lo[q], m) : mod_split(hi[q], lo[q], m);
+    }
 }
 
-template <int B, int CW>
-static void sell_b(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
-                   hipStream_t st) {
-    const uint64_t threads = M.rows * (B / CW);
+template <int B, int CW, int LANES>
+static void sell_launch(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
+                        hipStream_t st) {
+    const uint64_t threads = M.rows * (B / CW) * LANES;
     const dim3 grid(grid_for(threads)), blk(256);
     const uint2* ent = reinterpret_cast<const uint2*>(M.ent);
     if (m.p < 65536u)
-        hipLaunchKernelGGL((k_sell_spmm<B, CW, true>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
+        hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, true>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
     else
-        hipLaunchKernelGGL((k_sell_spmm<B, CW, false>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
+        hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, false>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
+}
+
+// lanes per row: enough threads to fill the chip (about 2^20) when rows are
+// few, never more than the map's mean padded row width; 1 when B / CW is not
+// a power of two (a row's threads must stay inside one wavefront)
+template <int B, int CW>
+static void sell_b(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
+                   hipStream_t st) {
+    constexpr int G = B / CW;
+    int lanes = 1;
+    if ((G & (G - 1)) == 0) {
+        const double width = M.rows ? (double)M.padded / M.rows : 0.0;
+        while (lanes < 8 && (double)M.rows * G * lanes < (double)(1 << 20) && 2.0 * lanes <= width)
+            lanes *= 2;
+    }
+    if constexpr ((G & (G - 1)) == 0 && G <= 8) {
+        if (lanes == 8) return sell_launch<B, CW, 8>(M, in, out, m, st);
+        if (lanes == 4) return sell_launch<B, CW, 4>(M, in, out, m, st);
+        if (lanes == 2) return sell_launch<B, CW, 2>(M, in, out, m, st);
+    }
+    sell_launch<B, CW, 1>(M, in, out, m, st);
 }
 
 void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
                       const ModP& m, hipStream_t st) {
     if (M.rows == 0) return;
-    require(M.rows * b / b == M.rows && M.rows < (1ULL << 40), "sparse map too large");
+    require(M.rows < (1ULL << 40), "sparse map too large");
     // 16-byte gathers need 16-byte aligned rows
     const bool vec = b % 4 == 0 && (reinterpret_cast<uintptr_t>(in) % 16) == 0;
     switch (b) {

@@ -120,9 +120,15 @@ struct pgw_op {
     uint64_t y_rows = 0;
     uint32_t* d_y = nullptr;
     uint32_t* d_t2 = nullptr;
-    // timing of the sweep kernels (pgw_bench_apply)
+    // per-stage timing of the apply chain (pgw_bench_apply, pgw_bench_stages):
+    // while `profile` is set every stage launch is bracketed by two events
+    struct StageEvent {
+        int stage;
+        double bytes;  // algorithmic bytes of the launch
+        hipEvent_t e0, e1;
+    };
     bool profile = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> sweep_events;
+    std::vector<StageEvent> stage_events;
 };
 
 namespace pgw {

@@ -85,6 +85,10 @@ def lib():
         f.argtypes = [_c.c_void_p, _u32p, _u32p, sz]
     L.orc_sweep_partial.restype = None
     L.orc_sweep_partial.argtypes = [_c.c_void_p, _c.c_int, _u32p, _u32p, sz, sz, sz]
+    L.orc_elementwise_sample.restype = None
+    L.orc_elementwise_sample.argtypes = [_c.c_void_p, _u32p, sz, sz]
+    L.orc_set_threads.restype = None
+    L.orc_set_threads.argtypes = [_c.c_int]
     L.orc_sweep_rows.restype = sz
     L.orc_sweep_rows.argtypes = [_c.c_void_p, _c.c_int]
     L.orc_gram_upper.restype = None
