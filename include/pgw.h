@@ -253,28 +253,29 @@ int pgw_bench_stages(pgw_op* op, uint32_t* d_a, uint32_t* d_b, uint32_t iters, d
  * The reference runs one sector per process, one after another
  * (scripts/reproduce.py:235-276 launching cuda/cuprym_cyclic.cu:2316-2461
  * per sector).  A batch holds one job per operator (each job is exactly the
- * pgw_sequence_run loop of that operator) and advances the unfinished jobs
- * by whole rounds.  One replay advances every job by its quota of rounds:
- * 1, or with PGW_BATCH_BALANCE about (slowest job's round time / this job's
- * round time), at most 8, from a cost model of the operator, so that small
- * sectors do not wait on large ones.  With PGW_BATCH_GRAPH one replay is
- * captured once into a HIP graph that fans the jobs out over `nstreams`
- * streams, replayed while the jobs and quotas are unchanged and recaptured
- * when they change.  Sequence positions live in a device counter per job,
- * so the same graph serves every round.  Each job's sequence and final
- * block are identical to pgw_sequence_run on the same operator and V.
- * pgw_batch_add / _poll / _result may be called from another thread than
- * pgw_batch_step (jobs are staged on a stream of their own).
+ * pgw_sequence_run loop of that operator); one replay advances every
+ * unfinished job by one round (two block-applies, four projections), all on
+ * the batch's single stream.  With PGW_BATCH_GROUPED, jobs of the synthetic
+ * operator on the MFMA path that share block width, limb count, modulus
+ * class, row-mixing rounds and tile configurations (7-9) form a group, and
+ * every stage of the round is ONE launch for the whole group (job j owns a
+ * block range of each grid); other jobs run their own launches.  With
+ * PGW_BATCH_GRAPH a round is captured once into a single-stream HIP graph
+ * and replayed while the set of jobs is unchanged.  Sequence positions live
+ * in a device counter per job, so the same launches serve every round.  Each
+ * job's sequence and final block are identical to pgw_sequence_run on the
+ * same operator and V.  pgw_batch_add / _poll / _result may be called from
+ * another thread than pgw_batch_step (jobs are staged on a stream of their
+ * own).
  * ---------------------------------------------------------------------- */
 typedef struct pgw_batch pgw_batch;
 
 enum pgw_batch_flags {
-    PGW_BATCH_GRAPH = 1,    /* replay captured HIP graphs (else launch every replay) */
-    PGW_BATCH_BALANCE = 2   /* per-job round quotas by modelled cost (else 1 each) */
+    PGW_BATCH_GRAPH = 1,    /* replay a captured graph of the round (else launch every replay) */
+    PGW_BATCH_GROUPED = 2   /* one launch per stage for each group of compatible jobs */
 };
 
-/* nstreams = 0 picks 8; flags: pgw_batch_flags. */
-int pgw_batch_create(pgw_ctx* ctx, uint32_t nstreams, uint32_t flags, pgw_batch** out);
+int pgw_batch_create(pgw_ctx* ctx, uint32_t flags, pgw_batch** out);
 /* Adds a job: h_v (N x b) is copied, seq[0] = V^T V is computed at once.
  * An operator may carry at most one unfinished job.  *job receives its id. */
 int pgw_batch_add(pgw_batch* b, pgw_op* op, const uint32_t* h_v, uint64_t max_len,
@@ -293,6 +294,8 @@ int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_cu
 int pgw_batch_cancel(pgw_batch* b, uint64_t job);
 /* Number of graph captures so far (instrumentation). */
 int pgw_batch_graph_builds(pgw_batch* b, uint64_t* builds);
+/* Groups of the current plan and the jobs they hold (instrumentation). */
+int pgw_batch_groups(pgw_batch* b, uint64_t* groups, uint64_t* grouped_jobs);
 int pgw_batch_destroy(pgw_batch* b);
 
 /* ------------------------------------------------------------------------
@@ -304,8 +307,9 @@ int pgw_batch_destroy(pgw_batch* b);
  * FIFO: the head of the queue is admitted when its footprint fits in
  * `hbm_budget` bytes next to the running jobs and fewer than `max_active`
  * jobs run; a job whose footprint exceeds the whole budget fails at once.
- * Between batch steps finished jobs leave and admitted jobs join.  Results
- * of a job do not depend on what else runs with it.
+ * Between batch steps (grouped, graph-replayed rounds) finished jobs leave
+ * and admitted jobs join.  Results of a job do not depend on what else runs
+ * with it.
  * ---------------------------------------------------------------------- */
 typedef struct pgw_server pgw_server;
 

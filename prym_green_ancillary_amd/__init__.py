@@ -100,7 +100,7 @@ EXPORTED = [
     "pgw_operator_info", "pgw_apply_square", "pgw_apply_forward", "pgw_gram_upper",
     "pgw_sequence_run", "pgw_kernel_vectors", "pgw_bench_apply", "pgw_operator_autotune",
     "pgw_batch_create", "pgw_batch_add", "pgw_batch_step", "pgw_batch_poll", "pgw_batch_result",
-    "pgw_batch_cancel", "pgw_batch_graph_builds", "pgw_batch_destroy", "pgw_server_create",
+    "pgw_batch_cancel", "pgw_batch_graph_builds", "pgw_batch_groups", "pgw_batch_destroy", "pgw_server_create",
     "pgw_server_submit", "pgw_server_wait", "pgw_server_result", "pgw_server_stats",
     "pgw_server_destroy", "pgw_job_footprint", "pgw_rank_create", "pgw_rank_step",
     "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
@@ -155,7 +155,8 @@ def lib():
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u32)]),
         "pgw_operator_autotune": (I, [vp, u32, vp, vp]),
         "pgw_bench_stages": (I, [vp, vp, vp, u32, vp, vp, vp]),
-        "pgw_batch_create": (I, [vp, u32, u32, ctypes.POINTER(vp)]),
+        "pgw_batch_create": (I, [vp, u32, ctypes.POINTER(vp)]),
+        "pgw_batch_groups": (I, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "pgw_batch_add": (I, [vp, vp, vp, u64, ctypes.POINTER(u64)]),
         "pgw_batch_step": (I, [vp, u32, ctypes.POINTER(u64)]),
         "pgw_batch_poll": (I, [vp, u64, ctypes.POINTER(I)]),
@@ -492,18 +493,25 @@ class Operator:
 
 
 class Batch:
-    """Many sector sequence loops advanced together (pgw_batch_*)."""
+    """Many sector sequence loops advanced together (pgw_batch_*): one round
+    of every job per replay on one stream; compatible MFMA jobs share one
+    launch per stage (grouped), a round replays as a captured graph."""
 
-    GRAPH, BALANCE = 1, 2
+    GRAPH, GROUPED = 1, 2
 
-    def __init__(self, ctx: Context, nstreams: int = 8, use_graph: bool = True,
-                 balance: bool = False):
+    def __init__(self, ctx: Context, use_graph: bool = True, grouped: bool = True):
         self.ctx = ctx
         h = ctypes.c_void_p()
-        flags = (self.GRAPH if use_graph else 0) | (self.BALANCE if balance else 0)
-        _check(lib().pgw_batch_create(ctx.h, nstreams, flags, ctypes.byref(h)))
+        flags = (self.GRAPH if use_graph else 0) | (self.GROUPED if grouped else 0)
+        _check(lib().pgw_batch_create(ctx.h, flags, ctypes.byref(h)))
         self.h = h
         self.jobs = {}
+
+    def groups(self):
+        """(groups, grouped jobs) of the current plan."""
+        g, n = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().pgw_batch_groups(self.h, ctypes.byref(g), ctypes.byref(n)))
+        return g.value, n.value
 
     def add(self, op: Operator, v, max_len: int) -> int:
         v = _u32(v).reshape(-1)

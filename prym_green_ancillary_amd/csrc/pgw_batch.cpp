@@ -15,6 +15,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <string>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -28,10 +29,19 @@ using namespace pgw;
 // ===========================================================================
 // batches
 // ===========================================================================
+//
+// Every replay advances each unfinished job by one round on ONE stream.  Jobs
+// of the synthetic operator on the MFMA path whose block width, limb count,
+// modulus class, row-mixing rounds and tile configurations agree form a
+// group: each stage of the round (digit split, the four sweeps, row mixing,
+// Gram projections, position counters) is one launch for the whole group,
+// over a grid in which job j owns a contiguous block range.  Other jobs run
+// their own launches after the groups'.  With PGW_BATCH_GRAPH a round is
+// captured once into a single-stream graph and replayed while the set of
+// jobs is unchanged.
 
 namespace {
 
-constexpr uint32_t kMaxQuota = 8;       // rounds of one job inside one graph replay
 constexpr unsigned kUploadStreams = 4;  // concurrent pgw_batch_add calls staging V
 
 struct BatchJob {
@@ -43,7 +53,6 @@ struct BatchJob {
     uint32_t* d_pos = nullptr;
     uint64_t L = 0, rounds_total = 0, rounds_done = 0;
     bool done = false;
-    double cost = 0.0;  // modelled seconds per round
 
     void release() {
         for (uint32_t* p : {d_B, d_D, d_seq, d_pos})
@@ -52,29 +61,10 @@ struct BatchJob {
     }
 };
 
-// Modelled device time of one round (two block-applies, four projections):
-// table-contraction work at the rate each path reaches on a large sector,
-// the elementwise passes at HBM speed, and a floor for ~24 kernel launches.
-double round_cost(const pgw_op* op) {
-    const double b = op->b;
-    double s = 0.0;
-    if (op->kind == PGW_KIND_SYNTHETIC) {
-        const double macs = 4.0 * (double)op->u_tgt * (op->n - op->k + 1) * op->Mp * op->M * b;
-        if (op->method == PGW_METHOD_MFMA)
-            s = macs * op->nl * op->nl / 4.0e14;
-        else
-            s = macs / 4.0e12;
-        s += 40.0 * (op->rows + op->cols + op->mid) * b * 4.0 / 4.0e12;
-    } else {
-        s = 200.0 * (op->rows + op->cols + op->mid) * b * 4.0 / 2.0e12;
-    }
-    return 2.0 * s + 100e-6;
-}
-
-// One round of the sequence loop for a job (cuprym_cyclic.cu:2439-2449):
-// D = S B; seq[pos] = B^T D, seq[pos+1] = D^T D; B = S D; seq[pos+2] = B^T D,
-// seq[pos+3] = B^T B; pos += 4.  Positions are read on the device, so the
-// same launches (or graph) serve every round.
+// One round of the sequence loop for a job on its own (cuprym_cyclic.cu:
+// 2439-2449): D = S B; seq[pos] = B^T D, seq[pos+1] = D^T D; B = S D;
+// seq[pos+2] = B^T D, seq[pos+3] = B^T B; pos += 4.  Positions are read on the
+// device, so the same launches (or graph) serve every round.
 void enqueue_round(const BatchJob& j, hipStream_t st) {
     pgw_op* op = j.op;
     const uint64_t ut = (uint64_t)op->b * (op->b + 1) / 2;
@@ -87,10 +77,178 @@ void enqueue_round(const BatchJob& j, hipStream_t st) {
     launch_advance_pos(j.d_pos, 4, st);
 }
 
-struct Slot {
-    uint64_t id;
-    uint32_t quota;
-    bool operator==(const Slot& o) const { return id == o.id && quota == o.quota; }
+// grouping key of a job, or an empty string when it runs on its own
+std::string group_key(const pgw_op* op) {
+    if (op->kind != PGW_KIND_SYNTHETIC || op->method != PGW_METHOD_MFMA) return "";
+    for (int w = 0; w < 4; ++w)
+        if (op->mfma_cfg[w] < 7 || op->mfma_cfg[w] > 9) return "";
+    std::string k = std::to_string(op->b) + "/" + std::to_string(op->nl) + "/" +
+                    (op->p == 0x7fffffffu ? "m31" : "bar") + "/" + std::to_string(op->rounds);
+    for (int w = 0; w < 4; ++w) k += "/" + std::to_string(op->mfma_cfg[w]);
+    return k;
+}
+
+// A device array of per-job launch records and their block offsets.
+template <typename T>
+struct GroupArrays {
+    T* d_jobs = nullptr;
+    uint32_t* d_first = nullptr;
+    uint64_t blocks = 0;
+};
+
+// The stage launches of one group.
+struct Group {
+    uint32_t b = 0, nl = 0, rounds = 0;
+    bool m31 = false;
+    int cfg[4] = {8, 8, 8, 8};
+    std::vector<uint64_t> ids;
+    std::vector<void*> allocs;
+    GroupArrays<LimbJob> split_B, split_D, split_mid, split_tgt;
+    GroupArrays<MfmaSweepArgs> sweep[3], sweep3_D, sweep3_B;
+    GroupArrays<MixJob> mix;
+    GroupArrays<GramJob> gram;
+    uint32_t* d_gram_fin = nullptr;  // finish-pass block offsets (P per job and product)
+    uint64_t gram_fin_blocks = 0;
+    uint32_t** d_pos = nullptr;
+
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+    }
+
+    template <typename T>
+    T* upload(const std::vector<T>& v) {
+        void* d = nullptr;
+        PGW_HIP(hipMalloc(&d, std::max<size_t>(1, v.size()) * sizeof(T)));
+        allocs.push_back(d);
+        if (!v.empty()) PGW_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return static_cast<T*>(d);
+    }
+
+    // records and block counts -> device arrays with offsets
+    template <typename T>
+    GroupArrays<T> make(const std::vector<T>& recs, const std::vector<uint64_t>& blocks) {
+        GroupArrays<T> g;
+        std::vector<uint32_t> first(recs.size() + 1, 0);
+        for (size_t q = 0; q < recs.size(); ++q) {
+            g.blocks += blocks[q];
+            require(g.blocks < (1ULL << 31), "grouped launch grid too large");
+            first[q + 1] = (uint32_t)g.blocks;
+        }
+        g.d_jobs = upload(recs);
+        g.d_first = upload(first);
+        return g;
+    }
+
+    void build(const std::vector<const BatchJob*>& js) {
+        const pgw_op* op0 = js[0]->op;
+        b = op0->b;
+        nl = op0->nl;
+        rounds = op0->rounds;
+        m31 = op0->p == 0x7fffffffu;
+        for (int w = 0; w < 4; ++w) cfg[w] = op0->mfma_cfg[w];
+        std::vector<LimbJob> sB, sD, smid, stgt;
+        std::vector<uint64_t> nsrc, nmid;
+        std::vector<MfmaSweepArgs> sw[3], s3D, s3B;
+        std::vector<uint64_t> nsw[3], ns3;
+        std::vector<MixJob> mx;
+        std::vector<uint64_t> nmx;
+        std::vector<GramJob> gr;
+        std::vector<uint64_t> ngr;
+        std::vector<uint32_t> fin(js.size() + 1, 0);
+        std::vector<uint32_t*> pos;
+        const uint32_t P = b * (b + 1) / 2;
+        for (const BatchJob* j : js) {
+            pgw_op* op = j->op;
+            ids.push_back(j->id);
+            const uint32_t ks = limb_stride(op->M), kt = limb_stride(op->Mp);
+            sB.push_back({j->d_B, op->d_dc, op->d_lp_src, op->u_src, op->M, ks, op->m});
+            sD.push_back({j->d_D, op->d_dc, op->d_lp_src, op->u_src, op->M, ks, op->m});
+            smid.push_back({op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, kt, op->m});
+            stgt.push_back({op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, ks, op->m});
+            nsrc.push_back(limb_group_blocks(sB.back(), b));
+            nmid.push_back(limb_group_blocks(smid.back(), b));
+            const int8_t* in_lp[3] = {op->d_lp_src, op->d_lp_mid, op->d_lp_src};
+            uint32_t* outs[3] = {op->d_mid, op->d_target, op->d_mid};
+            for (int w = 0; w < 3; ++w) {
+                MfmaSweepArgs a{};
+                const uint64_t blk = mfma_group_prepare(cfg[w], b, nl, sweep_desc_of(op, w), op->mt[w],
+                                                        in_lp[w], outs[w], nullptr, op->m, a);
+                sw[w].push_back(a);
+                nsw[w].push_back(blk);
+            }
+            MfmaSweepArgs a3{};
+            ns3.push_back(mfma_group_prepare(cfg[3], b, nl, sweep_desc_of(op, 3), op->mt[3], op->d_lp_mid,
+                                             j->d_D, op->d_dc, op->m, a3));
+            s3D.push_back(a3);
+            a3.out = j->d_B;
+            s3B.push_back(a3);
+            MixJob mj{op->d_pairs, op->pair_count, op->d_target, op->d_dr, op->unpaired, op->m};
+            mx.push_back(mj);
+            nmx.push_back(mix_group_blocks(mj, b));
+            const uint32_t gb = gram_group_blocks(op->cols);
+            gr.push_back({j->d_B, j->d_D, op->cols, op->d_gram, gb, j->d_seq, j->d_seq + P, j->d_pos, op->m});
+            ngr.push_back(gb);
+            pos.push_back(j->d_pos);
+        }
+        split_B = make(sB, nsrc);
+        split_D = make(sD, nsrc);
+        split_mid = make(smid, nmid);
+        split_tgt = make(stgt, nsrc);
+        for (int w = 0; w < 3; ++w) sweep[w] = make(sw[w], nsw[w]);
+        sweep3_D = make(s3D, ns3);
+        sweep3_B = make(s3B, ns3);
+        mix = make(mx, nmx);
+        gram = make(gr, ngr);
+        for (size_t q = 0; q < js.size(); ++q) fin[q + 1] = fin[q] + 2 * P;  // modes 1 and 2: two products
+        gram_fin_blocks = fin.back();
+        d_gram_fin = upload(fin);
+        d_pos = upload(pos);
+    }
+
+    static DenseSweep sweep_desc_of(const pgw_op* op, int which) {
+        DenseSweep s;
+        s.n = op->n;
+        s.k = op->k;
+        s.fwd = which == 0 || which == 2;
+        s.u_out = s.fwd ? op->u_tgt : op->u_src;
+        s.v_in = s.fwd ? op->u_src : op->u_tgt;
+        s.r_out = s.fwd ? op->Mp : op->M;
+        s.k_in = s.fwd ? op->M : op->Mp;
+        s.binom = op->ctx->d_binom;
+        s.inc = op->d_inc[s.fwd ? 0 : 1];
+        return s;
+    }
+
+    // one block-apply of every job: in -> out (B -> D or D -> B)
+    void apply(bool from_B, hipStream_t st) const {
+        const uint32_t n = (uint32_t)ids.size();
+        const GroupArrays<LimbJob>& src = from_B ? split_B : split_D;
+        launch_limb_split_group(src.d_jobs, src.d_first, n, src.blocks, b, nl, st);
+        mfma_group_launch(cfg[0], b, nl, true, sweep[0].d_jobs, sweep[0].d_first, n, sweep[0].blocks, st);
+        launch_limb_split_group(split_mid.d_jobs, split_mid.d_first, n, split_mid.blocks, b, nl, st);
+        mfma_group_launch(cfg[1], b, nl, false, sweep[1].d_jobs, sweep[1].d_first, n, sweep[1].blocks, st);
+        // P^T Dr P: rounds 0..R-2 forward, the fused last round, R-2..0 transposed
+        for (uint32_t r = 0; r + 1 < rounds; ++r)
+            launch_rowmix_group(mix.d_jobs, mix.d_first, n, mix.blocks, b, 0, m31, r, st);
+        launch_rowmix_group(mix.d_jobs, mix.d_first, n, mix.blocks, b, 2, m31, rounds - 1, st);
+        for (uint32_t r = rounds - 1; r-- > 0;)
+            launch_rowmix_group(mix.d_jobs, mix.d_first, n, mix.blocks, b, 1, m31, r, st);
+        launch_limb_split_group(split_tgt.d_jobs, split_tgt.d_first, n, split_tgt.blocks, b, nl, st);
+        mfma_group_launch(cfg[2], b, nl, true, sweep[2].d_jobs, sweep[2].d_first, n, sweep[2].blocks, st);
+        launch_limb_split_group(split_mid.d_jobs, split_mid.d_first, n, split_mid.blocks, b, nl, st);
+        const GroupArrays<MfmaSweepArgs>& s3 = from_B ? sweep3_D : sweep3_B;
+        mfma_group_launch(cfg[3], b, nl, false, s3.d_jobs, s3.d_first, n, s3.blocks, st);
+    }
+
+    void round(hipStream_t st) const {
+        const uint32_t n = (uint32_t)ids.size();
+        apply(true, st);
+        launch_gram_group(gram.d_jobs, gram.d_first, d_gram_fin, n, gram.blocks, gram_fin_blocks, b, 1, 0, st);
+        apply(false, st);
+        launch_gram_group(gram.d_jobs, gram.d_first, d_gram_fin, n, gram.blocks, gram_fin_blocks, b, 2, 2, st);
+        launch_advance_pos_group(d_pos, n, 4, st);
+    }
 };
 
 }  // namespace
@@ -98,19 +256,21 @@ struct Slot {
 struct pgw_batch {
     pgw_ctx* ctx = nullptr;
     uint32_t flags = 0;
-    std::vector<hipStream_t> streams;
+    hipStream_t stream = nullptr;  // every replay runs here
     hipStream_t upload[kUploadStreams] = {};
     std::mutex upload_mtx[kUploadStreams];
     std::atomic<unsigned> next_upload{0};
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> joins;
-    // `mtx` guards the job table; `step_mtx` serialises steps with cancel/destroy
+    // `mtx` guards the job table; `step_mtx` serialises steps with result,
+    // cancel and destroy
     std::mutex mtx, step_mtx;
     std::map<uint64_t, BatchJob> jobs;
     uint64_t next_id = 1;
+    // the plan of the current set of unfinished jobs (under step_mtx)
+    std::vector<uint64_t> plan_key;
+    std::vector<Group> groups;
+    std::vector<const BatchJob*> solo;  // map nodes stay put until result/cancel drop the plan
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    std::vector<Slot> graph_key;  // (job, rounds per replay) the graph was captured for
     uint64_t graph_builds = 0;
 
     void drop_graph() {
@@ -118,78 +278,92 @@ struct pgw_batch {
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
-        graph_key.clear();
     }
 
-    // every slot's rounds, fanned out over the streams and joined on streams[0]
-    void enqueue_all(const std::vector<Slot>& key, const std::vector<const BatchJob*>& js) {
-        hipStream_t s0 = streams[0];
-        const size_t S = std::min(streams.size(), js.size());
-        PGW_HIP(hipEventRecord(fork, s0));
-        for (size_t q = 1; q < S; ++q) PGW_HIP(hipStreamWaitEvent(streams[q], fork, 0));
-        // largest jobs first, so that the longest chains start earliest
-        for (size_t q = 0; q < js.size(); ++q)
-            for (uint32_t r = 0; r < key[q].quota; ++r) enqueue_round(*js[q], streams[q % S]);
-        for (size_t q = 1; q < S; ++q) {
-            PGW_HIP(hipEventRecord(joins[q], streams[q]));
-            PGW_HIP(hipStreamWaitEvent(s0, joins[q], 0));
+    // under step_mtx, with nothing of the plan in flight
+    void drop_plan() {
+        drop_graph();
+        std::lock_guard<std::mutex> cl(ctx->capture_mtx);
+        for (Group& g : groups) g.release();
+        groups.clear();
+        solo.clear();
+        plan_key.clear();
+    }
+
+    bool plan_uses(uint64_t id) const {
+        return std::find(plan_key.begin(), plan_key.end(), id) != plan_key.end();
+    }
+
+    void build_plan(const std::vector<uint64_t>& key, const std::vector<const BatchJob*>& js) {
+        drop_plan();
+        std::map<std::string, std::vector<const BatchJob*>> classes;
+        for (const BatchJob* j : js) {
+            const std::string k = (flags & PGW_BATCH_GROUPED) ? group_key(j->op) : "";
+            if (k.empty() || j->op->rounds == 0 || j->op->pair_count == 0) solo.push_back(j);
+            else classes[k].push_back(j);
         }
+        std::lock_guard<std::mutex> cl(ctx->capture_mtx);
+        for (auto& kv : classes) {
+            groups.emplace_back();
+            try {
+                groups.back().build(kv.second);
+            } catch (...) {
+                groups.back().release();
+                groups.pop_back();
+                throw;
+            }
+        }
+        plan_key = key;
+    }
+
+    void enqueue_all() {
+        for (const Group& g : groups) g.round(stream);
+        for (const BatchJob* j : solo) enqueue_round(*j, stream);
     }
 
     uint64_t step(uint32_t max_replays) {
         std::lock_guard<std::mutex> sl(step_mtx);
-        std::vector<Slot> key;
-        std::vector<BatchJob*> js;
+        std::vector<uint64_t> key;
+        std::vector<const BatchJob*> js;
+        uint64_t reps = std::max<uint32_t>(1, max_replays);
         {
             std::lock_guard<std::mutex> lk(mtx);
             for (auto& kv : jobs)
-                if (!kv.second.done) js.push_back(&kv.second);
-            if (js.empty()) return 0;
-            std::stable_sort(js.begin(), js.end(),
-                             [](const BatchJob* a, const BatchJob* b) { return a->cost > b->cost; });
-            const double top = js.front()->cost;
-            for (BatchJob* j : js) {
-                const uint64_t left = j->rounds_total - j->rounds_done;
-                uint64_t q = 1;
-                if (flags & PGW_BATCH_BALANCE) q = (uint64_t)std::floor(top / j->cost);
-                q = std::max<uint64_t>(1, std::min<uint64_t>({q, kMaxQuota, left}));
-                key.push_back({j->id, (uint32_t)q});
-            }
+                if (!kv.second.done) {
+                    key.push_back(kv.first);
+                    js.push_back(&kv.second);
+                    reps = std::min<uint64_t>(reps, kv.second.rounds_total - kv.second.rounds_done);
+                }
         }
-        // replays while every job still has a full quota left
-        uint64_t reps = std::max<uint32_t>(1, max_replays);
-        for (size_t q = 0; q < js.size(); ++q)
-            reps = std::min<uint64_t>(reps, (js[q]->rounds_total - js[q]->rounds_done) / key[q].quota);
-        std::vector<const BatchJob*> cjs(js.begin(), js.end());
-        hipStream_t s0 = streams[0];
+        if (js.empty()) return 0;
+        if (key != plan_key) build_plan(key, js);
         if (flags & PGW_BATCH_GRAPH) {
-            if (!(key == graph_key) || !exec) {
+            if (!exec) {
                 std::lock_guard<std::mutex> cl(ctx->capture_mtx);
-                drop_graph();
-                PGW_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+                PGW_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 try {
-                    enqueue_all(key, cjs);
+                    enqueue_all();
                 } catch (...) {
                     hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(s0, &g);
+                    (void)hipStreamEndCapture(stream, &g);
                     if (g) (void)hipGraphDestroy(g);
                     throw;
                 }
-                PGW_HIP(hipStreamEndCapture(s0, &graph));
+                PGW_HIP(hipStreamEndCapture(stream, &graph));
                 PGW_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                graph_key = key;
                 ++graph_builds;
             }
-            for (uint64_t r = 0; r < reps; ++r) PGW_HIP(hipGraphLaunch(exec, s0));
+            for (uint64_t r = 0; r < reps; ++r) PGW_HIP(hipGraphLaunch(exec, stream));
         } else {
-            for (uint64_t r = 0; r < reps; ++r) enqueue_all(key, cjs);
+            for (uint64_t r = 0; r < reps; ++r) enqueue_all();
         }
-        PGW_HIP(hipStreamSynchronize(s0));
+        PGW_HIP(hipStreamSynchronize(stream));
         std::lock_guard<std::mutex> lk(mtx);
         uint64_t active = 0;
-        for (size_t q = 0; q < js.size(); ++q) {
-            js[q]->rounds_done += reps * key[q].quota;
-            if (js[q]->rounds_done >= js[q]->rounds_total) js[q]->done = true;
+        for (uint64_t id : key) {
+            BatchJob& j = jobs.at(id);
+            j.rounds_done += reps;
+            if (j.rounds_done >= j.rounds_total) j.done = true;
         }
         for (auto& kv : jobs) active += kv.second.done ? 0 : 1;
         return active;
@@ -198,32 +372,23 @@ struct pgw_batch {
 
 extern "C" {
 
-int pgw_batch_create(pgw_ctx* ctx, uint32_t nstreams, uint32_t flags, pgw_batch** out) {
+int pgw_batch_create(pgw_ctx* ctx, uint32_t flags, pgw_batch** out) {
     pgw_batch* b = nullptr;
     int rc = guarded([&] {
         require(ctx && out, "null context or output pointer");
-        require((flags & ~(uint32_t)(PGW_BATCH_GRAPH | PGW_BATCH_BALANCE)) == 0, "unknown batch flags");
+        require((flags & ~(uint32_t)(PGW_BATCH_GRAPH | PGW_BATCH_GROUPED)) == 0, "unknown batch flags");
         PGW_HIP(hipSetDevice(ctx->device));
         b = new pgw_batch;
         b->ctx = ctx;
         b->flags = flags;
-        if (nstreams == 0) nstreams = 8;
-        b->streams.resize(nstreams, nullptr);
-        b->joins.resize(nstreams, nullptr);
-        for (uint32_t q = 0; q < nstreams; ++q) {
-            PGW_HIP(hipStreamCreateWithFlags(&b->streams[q], hipStreamNonBlocking));
-            PGW_HIP(hipEventCreateWithFlags(&b->joins[q], hipEventDisableTiming));
-        }
+        PGW_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         for (auto& u : b->upload) PGW_HIP(hipStreamCreateWithFlags(&u, hipStreamNonBlocking));
-        PGW_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
         ctx->live.fetch_add(1);
         *out = b;
     });
     if (rc != PGW_OK && b) {
-        for (auto s : b->streams) if (s) (void)hipStreamDestroy(s);
-        for (auto e : b->joins) if (e) (void)hipEventDestroy(e);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
         for (auto u : b->upload) if (u) (void)hipStreamDestroy(u);
-        if (b->fork) (void)hipEventDestroy(b->fork);
         delete b;
     }
     return rc;
@@ -251,7 +416,6 @@ int pgw_batch_add(pgw_batch* b, pgw_op* op, const uint32_t* h_v, uint64_t max_le
         j.op = op;
         j.rounds_total = (max_len - 1 + 3) / 4;
         j.L = 1 + 4 * j.rounds_total;
-        j.cost = round_cost(op);
         PGW_HIP(hipSetDevice(b->ctx->device));
         {
             std::lock_guard<std::mutex> cl(b->ctx->capture_mtx);
@@ -259,16 +423,21 @@ int pgw_batch_add(pgw_batch* b, pgw_op* op, const uint32_t* h_v, uint64_t max_le
             PGW_HIP(hipMalloc(&j.d_D, N * bb * sizeof(uint32_t)));
             PGW_HIP(hipMalloc(&j.d_seq, j.L * ut * sizeof(uint32_t)));
             PGW_HIP(hipMalloc(&j.d_pos, sizeof(uint32_t)));
+            static const uint32_t kFirstPosition = 1;  // seq[0] = V^T V below
+            PGW_HIP(hipMemcpy(j.d_pos, &kFirstPosition, sizeof(uint32_t), hipMemcpyHostToDevice));
         }
         // staged on one of the batch's upload streams: steps of other jobs keep running
         const unsigned u = b->next_upload.fetch_add(1) % kUploadStreams;
         std::lock_guard<std::mutex> ul(b->upload_mtx[u]);
         std::lock_guard<std::mutex> cl(b->ctx->capture_mtx);
         hipStream_t up = b->upload[u];
-        const uint32_t one = 1;
         PGW_HIP(hipMemcpyAsync(j.d_B, h_v, N * bb * sizeof(uint32_t), hipMemcpyHostToDevice, up));
-        PGW_HIP(hipMemcpyAsync(j.d_pos, &one, sizeof(uint32_t), hipMemcpyHostToDevice, up));
-        launch_gram(j.d_B, j.d_B, N, op->b, 0, j.d_seq, nullptr, op->d_gram, op->m, up);  // V^T V
+        try {
+            launch_gram(j.d_B, j.d_B, N, op->b, 0, j.d_seq, nullptr, op->d_gram, op->m, up);  // V^T V
+        } catch (...) {
+            (void)hipStreamSynchronize(up);  // the copy still reads h_v
+            throw;
+        }
         PGW_HIP(hipStreamSynchronize(up));
         if (j.rounds_total == 0) j.done = true;
         std::lock_guard<std::mutex> lk(b->mtx);
@@ -307,6 +476,8 @@ int pgw_batch_poll(pgw_batch* b, uint64_t job, int* done) {
 int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_curv, uint64_t* len) {
     return guarded([&] {
         require(b && h_seq && h_curv, "null batch or buffer");
+        // lock order: step_mtx, then the context's capture_mtx, then mtx
+        std::lock_guard<std::mutex> sl(b->step_mtx);
         BatchJob j;
         {
             std::lock_guard<std::mutex> lk(b->mtx);
@@ -316,6 +487,8 @@ int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_cu
             j = it->second;
             b->jobs.erase(it);
         }
+        // the job's buffers leave no live plan or graph behind them
+        if (b->plan_uses(job)) b->drop_plan();
         std::lock_guard<std::mutex> cl(b->ctx->capture_mtx);
         struct Free {
             BatchJob& j;
@@ -332,15 +505,17 @@ int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_cu
 int pgw_batch_cancel(pgw_batch* b, uint64_t job) {
     return guarded([&] {
         require(b != nullptr, "null batch");
-        // lock order: step_mtx, then the context's capture_mtx, then mtx
         std::lock_guard<std::mutex> sl(b->step_mtx);
+        PGW_HIP(hipSetDevice(b->ctx->device));
+        PGW_HIP(hipStreamSynchronize(b->stream));
+        {
+            std::lock_guard<std::mutex> lk(b->mtx);
+            require(b->jobs.find(job) != b->jobs.end(), "unknown batch job");
+        }
+        if (b->plan_uses(job)) b->drop_plan();
         std::lock_guard<std::mutex> cl(b->ctx->capture_mtx);
         std::lock_guard<std::mutex> lk(b->mtx);
         auto it = b->jobs.find(job);
-        require(it != b->jobs.end(), "unknown batch job");
-        PGW_HIP(hipSetDevice(b->ctx->device));
-        for (auto s : b->streams) PGW_HIP(hipStreamSynchronize(s));
-        b->drop_graph();
         it->second.release();
         b->jobs.erase(it);
     });
@@ -354,20 +529,32 @@ int pgw_batch_graph_builds(pgw_batch* b, uint64_t* builds) {
     });
 }
 
+int pgw_batch_groups(pgw_batch* b, uint64_t* groups, uint64_t* grouped_jobs) {
+    return guarded([&] {
+        require(b != nullptr, "null batch");
+        std::lock_guard<std::mutex> sl(b->step_mtx);
+        uint64_t n = 0;
+        for (const Group& g : b->groups) n += g.ids.size();
+        if (groups) *groups = b->groups.size();
+        if (grouped_jobs) *grouped_jobs = n;
+    });
+}
+
 int pgw_batch_destroy(pgw_batch* b) {
     return guarded([&] {
         if (!b) return;
         {
             std::lock_guard<std::mutex> sl(b->step_mtx);
             PGW_HIP(hipSetDevice(b->ctx->device));
-            for (auto s : b->streams) (void)hipStreamSynchronize(s);
+            (void)hipStreamSynchronize(b->stream);
             for (auto u : b->upload) (void)hipStreamSynchronize(u);
-            b->drop_graph();
-            for (auto& kv : b->jobs) kv.second.release();
-            for (auto s : b->streams) (void)hipStreamDestroy(s);
-            for (auto e : b->joins) (void)hipEventDestroy(e);
+            b->drop_plan();
+            {
+                std::lock_guard<std::mutex> cl(b->ctx->capture_mtx);
+                for (auto& kv : b->jobs) kv.second.release();
+            }
+            (void)hipStreamDestroy(b->stream);
             for (auto u : b->upload) (void)hipStreamDestroy(u);
-            (void)hipEventDestroy(b->fork);
         }
         b->ctx->live.fetch_sub(1);
         delete b;
@@ -702,8 +889,7 @@ int pgw_server_create(pgw_ctx* ctx, uint64_t hbm_budget, uint32_t max_active,
         s->budget = hbm_budget;
         s->max_active = max_active;
         s->replays_per_step = replays_per_step ? replays_per_step : 1;
-        if (pgw_batch_create(ctx, std::min<uint32_t>(max_active, 8),
-                             PGW_BATCH_GRAPH | PGW_BATCH_BALANCE, &s->batch) != PGW_OK)
+        if (pgw_batch_create(ctx, PGW_BATCH_GRAPH | PGW_BATCH_GROUPED, &s->batch) != PGW_OK)
             throw Error(PGW_ERR_HIP, pgw_last_error());
         for (int w = 0; w < kServerWorkers; ++w) s->workers.emplace_back([s] { s->io_loop(); });
         s->stepper = std::thread([s] { s->step_loop(); });

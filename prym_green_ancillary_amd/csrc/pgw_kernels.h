@@ -22,6 +22,28 @@ struct MfmaTable {
     uint32_t rows = 0, k_in = 0, rows_pad = 0, k_pad = 0;
 };
 
+struct ShiftWeights {
+    uint32_t w[8];
+};
+
+// Kernel arguments of one MFMA sweep launch (pgw_mfma_sweep.h).
+struct MfmaSweepArgs {
+    const uint32_t* inc;    // incidence table of the sweep direction (n + 1 words per output panel)
+    const int8_t* tabl;
+    const int8_t* lp;
+    const int8_t* zero;     // 1 KB of zero bytes (staging loads that have no data)
+    uint32_t n, deg_out;
+    uint64_t u_out, v_in;
+    uint32_t r_out, r_pad, k_in, k_pad;
+    uint32_t row_slices;
+    uint64_t col_groups;
+    uint32_t xcd_affinity;  // 1: grid is 8 * ceil(col_groups / (8 / row_slices)) blocks
+    uint32_t* out;
+    const uint32_t* scale;
+    ModP m;
+    ShiftWeights sw;
+};
+
 // Launch description of one dense-table (r = 1) Koszul sweep.
 struct DenseSweep {
     bool fwd = true;          // forward gather (out: (k-1)-subsets) or transpose (out: k-subsets)
@@ -98,6 +120,76 @@ std::vector<uint32_t> mfma_incidence_table(uint32_t n, uint32_t deg_out, bool fw
 uint32_t limb_stride(uint32_t k_in);
 void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
                        uint32_t k_in, uint32_t b, uint32_t nl, const ModP& m, hipStream_t st);
+// Grouped launches (pgw_batch.cpp): the same stage of many jobs in one grid;
+// job j owns blocks [first[j], first[j+1]) of it, its arguments at jobs[j]
+// (both arrays in device memory).
+// The job of this block in a grouped launch (uniform across the block):
+// first[lo] <= blockIdx.x < first[lo + 1].  Ranges are multiples of 8 blocks
+// for the MFMA sweeps, so that a block's XCD (hardware block id mod 8) is its
+// local id mod 8.
+__device__ __forceinline__ uint32_t group_job(const uint32_t* __restrict__ first, uint32_t njobs) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (first[mid] <= blockIdx.x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+struct LimbJob {
+    const uint32_t* in;
+    const uint32_t* scale;
+    int8_t* lp;
+    uint64_t v_count;
+    uint32_t k_in, k_stride;
+    ModP m;
+};
+uint64_t limb_group_blocks(const LimbJob& j, uint32_t b);
+void launch_limb_split_group(const LimbJob* jobs, const uint32_t* first, uint32_t njobs,
+                             uint64_t blocks, uint32_t b, uint32_t nl, hipStream_t st);
+struct MixJob {
+    const pgw_rowmix_pair* pairs;  // round r at pairs + r * count
+    uint64_t count;
+    uint32_t* block;
+    const uint32_t* dr;
+    int64_t unpaired;              // row outside the last round's pairs, or -1
+    ModP m;
+};
+uint64_t mix_group_blocks(const MixJob& j, uint32_t b);
+// mode 0 forward round, 1 transposed round, 2 the fused P_r^T Dr P_r (plus
+// the unpaired row's Dr); m31: every job's p is 2^31 - 1
+void launch_rowmix_group(const MixJob* jobs, const uint32_t* first, uint32_t njobs, uint64_t blocks,
+                         uint32_t b, int mode, bool m31, uint32_t round, hipStream_t st);
+struct GramJob {
+    const uint32_t* x;
+    const uint32_t* y;
+    uint64_t rows;
+    uint64_t* part;    // the job's Gram scratch
+    uint32_t nblocks;  // its partial workgroups (gram_group_blocks)
+    uint32_t* out0;
+    uint32_t* out1;
+    const uint32_t* pos;
+    ModP m;
+};
+uint32_t gram_group_blocks(uint64_t rows);
+// partial pass over blocks [first_part[j], ...), finish pass over
+// [first_fin[j], ...) (P or 2P entries per job)
+void launch_gram_group(const GramJob* jobs, const uint32_t* first_part, const uint32_t* first_fin,
+                       uint32_t njobs, uint64_t part_blocks, uint64_t fin_blocks, uint32_t b, int mode,
+                       uint32_t pos_off, hipStream_t st);
+// *pos[j] += by for every job
+void launch_advance_pos_group(uint32_t* const* pos, uint32_t njobs, uint32_t by, hipStream_t st);
+
+// Grouped sweeps (pgw_batch.cpp): many jobs of one block width, limb count,
+// direction and register-A tile configuration (7-9) in one launch.  prepare
+// fills a job's arguments and returns its block count (a multiple of 8);
+// launch runs jobs[0, njobs) whose blocks start at first[j] (first[njobs] =
+// blocks), both arrays in device memory.
+uint64_t mfma_group_prepare(int cfg, uint32_t b, uint32_t nl, const DenseSweep& s, const MfmaTable& t,
+                            const int8_t* lp, uint32_t* out, const uint32_t* scale, const ModP& m,
+                            MfmaSweepArgs& a);
+void mfma_group_launch(int cfg, uint32_t b, uint32_t nl, bool fwd, const MfmaSweepArgs* jobs,
+                       const uint32_t* first, uint32_t njobs, uint64_t blocks, hipStream_t st);
 // cfg selects a tile configuration in [0, mfma_config_count())
 int mfma_config_count();
 void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,

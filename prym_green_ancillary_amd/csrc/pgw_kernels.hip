@@ -345,14 +345,8 @@ __device__ __forceinline__ uint32_t red63(uint64_t v, const ModP& m) {
 // of the same round: the 2x2 block P_r^T diag(dr_f, dr_s) P_r on each pair.
 // Every sum below is u*x + v*y with residues below 2^31, so below 2^63.
 template <int B, int MODE, bool M31>  // MODE 0 forward, 1 transpose, 2 fused middle
-__global__ __launch_bounds__(256) void k_rowmix(const pgw_rowmix_pair* __restrict__ pairs,
-                                                uint64_t count, uint32_t* __restrict__ block,
-                                                const uint32_t* __restrict__ dr, ModP m) {
-    constexpr int RPB = 256 / B;
-    const uint64_t q = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB || q >= count) return;
-    const pgw_rowmix_pair pr = pairs[q];
+__device__ __forceinline__ void rowmix_pair_one(const pgw_rowmix_pair& pr, uint32_t* __restrict__ block,
+                                                const uint32_t* __restrict__ dr, const ModP& m, int c) {
     const uint32_t p = m.p;
     const uint32_t opab = addmod(1u, red63<M31>((uint64_t)pr.a * pr.b, m), p);
     uint32_t* fx = block + (uint64_t)pr.first * B + c;
@@ -374,6 +368,37 @@ __global__ __launch_bounds__(256) void k_rowmix(const pgw_rowmix_pair* __restric
     }
     *fx = nx;
     *sy = ny;
+}
+
+template <int B, int MODE, bool M31>
+__global__ __launch_bounds__(256) void k_rowmix(const pgw_rowmix_pair* __restrict__ pairs,
+                                                uint64_t count, uint32_t* __restrict__ block,
+                                                const uint32_t* __restrict__ dr, ModP m) {
+    constexpr int RPB = 256 / B;
+    const uint64_t q = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
+    const int c = threadIdx.x % B;
+    if (threadIdx.x / B >= RPB || q >= count) return;
+    rowmix_pair_one<B, MODE, M31>(pairs[q], block, dr, m, c);
+}
+
+// many jobs' row-mixing rounds in one launch; slot `count` of a job (one past
+// its pairs) scales its unpaired row by Dr in the fused middle pass
+template <int B, int MODE, bool M31>
+__global__ __launch_bounds__(256) void k_rowmix_group(const MixJob* __restrict__ jobs,
+                                                      const uint32_t* __restrict__ first,
+                                                      uint32_t njobs, uint32_t round) {
+    constexpr int RPB = 256 / B;
+    const uint32_t j = group_job(first, njobs);
+    const MixJob& J = jobs[j];
+    const uint64_t q = (uint64_t)(blockIdx.x - first[j]) * RPB + threadIdx.x / B;
+    const int c = threadIdx.x % B;
+    if (threadIdx.x / B >= RPB) return;
+    if (q < J.count) {
+        rowmix_pair_one<B, MODE, M31>(J.pairs[(uint64_t)round * J.count + q], J.block, J.dr, J.m, c);
+    } else if (MODE == 2 && q == J.count && J.unpaired >= 0) {
+        uint32_t* x = J.block + (uint64_t)J.unpaired * B + c;
+        *x = mulmod(*x, J.dr[J.unpaired], J.m);
+    }
 }
 
 // the row left out of the last round's pairs (odd row count) is fixed by that
@@ -408,6 +433,35 @@ static void target_precond_b(uint32_t* block, const pgw_rowmix_pair* pairs, uint
                              const ModP& m, hipStream_t st) {
     if (m.p == 0x7fffffffu) target_precond_bm<B, true>(block, pairs, pc, rounds, dr, unpaired_row, m, st);
     else target_precond_bm<B, false>(block, pairs, pc, rounds, dr, unpaired_row, m, st);
+}
+
+uint64_t mix_group_blocks(const MixJob& j, uint32_t b) {
+    const uint64_t rpb = 256 / b;
+    return (j.count + 1 + rpb - 1) / rpb;  // one slot past the pairs: the unpaired row
+}
+
+template <int B>
+static void rowmix_group_b(const MixJob* jobs, const uint32_t* first, uint32_t njobs, uint64_t blocks,
+                           int mode, bool m31, uint32_t round, hipStream_t st) {
+    const dim3 grid((unsigned)blocks), blk(256);
+#define PGW_RG(MODE, M31)                                                                         \
+    hipLaunchKernelGGL((k_rowmix_group<B, MODE, M31>), grid, blk, 0, st, jobs, first, njobs, round)
+    if (m31) {
+        if (mode == 0) PGW_RG(0, true); else if (mode == 1) PGW_RG(1, true); else PGW_RG(2, true);
+    } else {
+        if (mode == 0) PGW_RG(0, false); else if (mode == 1) PGW_RG(1, false); else PGW_RG(2, false);
+    }
+#undef PGW_RG
+}
+
+void launch_rowmix_group(const MixJob* jobs, const uint32_t* first, uint32_t njobs, uint64_t blocks,
+                         uint32_t b, int mode, bool m31, uint32_t round, hipStream_t st) {
+    if (blocks == 0) return;
+    require(blocks < (1ULL << 31), "grouped row mixing grid too large");
+#define PGW_CALL(BB) rowmix_group_b<BB>(jobs, first, njobs, blocks, mode, m31, round, st)
+    PGW_SWITCH_B(b, PGW_CALL)
+#undef PGW_CALL
+    PGW_CHECK_LAUNCH("grouped row mixing");
 }
 
 void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pair_count,
@@ -449,9 +503,10 @@ size_t gram_scratch_bytes(uint32_t b) {
 }
 
 template <int B>
-__global__ __launch_bounds__(256) void k_gram_partial(const uint32_t* __restrict__ x,
-                                                      const uint32_t* __restrict__ y, uint64_t rows,
-                                                      int mode, uint64_t* __restrict__ part) {
+__device__ __forceinline__ void gram_partial_one(const uint32_t* __restrict__ x,
+                                                 const uint32_t* __restrict__ y, uint64_t rows,
+                                                 int mode, uint64_t* __restrict__ part, uint32_t block,
+                                                 uint32_t nblocks) {
     constexpr int P = B * (B + 1) / 2;
     constexpr int L = 256 / P;  // row lanes (P <= 136)
     __shared__ uint32_t sx[kGramRows * B];
@@ -467,8 +522,7 @@ __global__ __launch_bounds__(256) void k_gram_partial(const uint32_t* __restrict
     }
     const int j = i + rem;
     uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
-    for (uint64_t r0 = (uint64_t)blockIdx.x * kGramRows; r0 < rows;
-         r0 += (uint64_t)gridDim.x * kGramRows) {
+    for (uint64_t r0 = (uint64_t)block * kGramRows; r0 < rows; r0 += (uint64_t)nblocks * kGramRows) {
         const uint64_t nrow = min((uint64_t)kGramRows, rows - r0);
         __syncthreads();
         for (int e = t; e < kGramRows * B; e += 256) {
@@ -502,29 +556,44 @@ __global__ __launch_bounds__(256) void k_gram_partial(const uint32_t* __restrict
         uint64_t a[4] = {0, 0, 0, 0};
         for (int l = 0; l < L; ++l)
             for (int q = 0; q < 4; ++q) a[q] += red[q][l * P + t];
-        const uint64_t nb = gridDim.x;
-        uint64_t* o = part + ((uint64_t)t * nb + blockIdx.x) * 2;
+        const uint64_t nb = nblocks;
+        uint64_t* o = part + ((uint64_t)t * nb + block) * 2;
         o[0] = a[0];
         o[1] = a[1];
         if (mode) {
-            uint64_t* o1 = part + ((uint64_t)(P + t) * nb + blockIdx.x) * 2;
+            uint64_t* o1 = part + ((uint64_t)(P + t) * nb + block) * 2;
             o1[0] = a[2];
             o1[1] = a[3];
         }
     }
 }
 
+template <int B>
+__global__ __launch_bounds__(256) void k_gram_partial(const uint32_t* __restrict__ x,
+                                                      const uint32_t* __restrict__ y, uint64_t rows,
+                                                      int mode, uint64_t* __restrict__ part) {
+    gram_partial_one<B>(x, y, rows, mode, part, blockIdx.x, gridDim.x);
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void k_gram_partial_group(const GramJob* __restrict__ jobs,
+                                                            const uint32_t* __restrict__ first,
+                                                            uint32_t njobs, int mode) {
+    const uint32_t j = group_job(first, njobs);
+    const GramJob& J = jobs[j];
+    gram_partial_one<B>(J.x, J.y, J.rows, mode, J.part, blockIdx.x - first[j], J.nblocks);
+}
+
 // One workgroup per output entry (mat * P + pair).  With `pos` set, out0/out1
 // are offset by ut * (*pos + pos_off) entries: the sequence position is read
 // on the device, so a captured graph of a whole round replays unchanged.
-__global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict__ part,
-                                                     unsigned nblocks, uint32_t P,
-                                                     uint32_t* __restrict__ out0,
-                                                     uint32_t* __restrict__ out1, ModP m,
-                                                     const uint32_t* __restrict__ pos,
-                                                     uint32_t pos_off) {
+__device__ __forceinline__ void gram_finish_one(const uint64_t* __restrict__ part, unsigned nblocks,
+                                                uint32_t P, uint32_t* __restrict__ out0,
+                                                uint32_t* __restrict__ out1, const ModP& m,
+                                                const uint32_t* __restrict__ pos, uint32_t pos_off,
+                                                uint32_t e) {
     __shared__ uint64_t s_lo[256], s_hi[256];
-    const uint32_t e = blockIdx.x, t = threadIdx.x;
+    const uint32_t t = threadIdx.x;
     uint64_t lo = 0, hi = 0;
     for (uint32_t g = t; g < nblocks; g += 256) {
         const uint64_t* o = part + ((uint64_t)e * nblocks + g) * 2;
@@ -549,6 +618,53 @@ __global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict__ part,
+                                                     unsigned nblocks, uint32_t P,
+                                                     uint32_t* __restrict__ out0,
+                                                     uint32_t* __restrict__ out1, ModP m,
+                                                     const uint32_t* __restrict__ pos,
+                                                     uint32_t pos_off) {
+    gram_finish_one(part, nblocks, P, out0, out1, m, pos, pos_off, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_gram_finish_group(const GramJob* __restrict__ jobs,
+                                                           const uint32_t* __restrict__ first,
+                                                           uint32_t njobs, uint32_t P,
+                                                           uint32_t pos_off) {
+    const uint32_t j = group_job(first, njobs);
+    const GramJob& J = jobs[j];
+    gram_finish_one(J.part, J.nblocks, P, J.out0, J.out1, J.m, J.pos, pos_off, blockIdx.x - first[j]);
+}
+
+__global__ void k_advance_pos_group(uint32_t* const* __restrict__ pos, uint32_t njobs, uint32_t by) {
+    for (uint32_t j = threadIdx.x; j < njobs; j += blockDim.x) *pos[j] += by;
+}
+
+void launch_advance_pos_group(uint32_t* const* pos, uint32_t njobs, uint32_t by, hipStream_t st) {
+    if (njobs == 0) return;
+    hipLaunchKernelGGL(k_advance_pos_group, dim3(1), dim3(256), 0, st, pos, njobs, by);
+    PGW_CHECK_LAUNCH("grouped advance position");
+}
+
+uint32_t gram_group_blocks(uint64_t rows) {
+    return (uint32_t)std::min<uint64_t>(kGramGrid, std::max<uint64_t>(1, (rows + kGramRowsPerBlock - 1) / kGramRowsPerBlock));
+}
+
+void launch_gram_group(const GramJob* jobs, const uint32_t* first_part, const uint32_t* first_fin,
+                       uint32_t njobs, uint64_t part_blocks, uint64_t fin_blocks, uint32_t b, int mode,
+                       uint32_t pos_off, hipStream_t st) {
+    if (njobs == 0) return;
+    require(part_blocks < (1ULL << 31) && fin_blocks < (1ULL << 31), "grouped Gram grid too large");
+#define PGW_CALL(BB)                                                                              \
+    hipLaunchKernelGGL(k_gram_partial_group<BB>, dim3((unsigned)part_blocks), dim3(256), 0, st,   \
+                       jobs, first_part, njobs, mode)
+    PGW_SWITCH_B(b, PGW_CALL)
+#undef PGW_CALL
+    hipLaunchKernelGGL(k_gram_finish_group, dim3((unsigned)fin_blocks), dim3(256), 0, st, jobs,
+                       first_fin, njobs, b * (b + 1) / 2, pos_off);
+    PGW_CHECK_LAUNCH("grouped gram");
+}
+
 __global__ void k_advance_pos(uint32_t* pos, uint32_t by) {
     if (threadIdx.x == 0) *pos += by;
 }
@@ -567,6 +683,7 @@ static void gram_launch(const uint32_t* x, const uint32_t* y, uint64_t rows, uin
     require(mode == 0 || out1 != nullptr, "Gram projection of two products needs both outputs");
     const unsigned grid = (unsigned)std::min<uint64_t>(
         max_grid, std::max<uint64_t>(1, (rows + kGramRowsPerBlock - 1) / kGramRowsPerBlock));
+    // (the grouped launch, gram_group_blocks, uses the same rule with kGramGrid)
     const uint32_t P = b * (b + 1) / 2;
 #define PGW_CALL(BB)                                                                           \
     hipLaunchKernelGGL(k_gram_partial<BB>, dim3(grid), dim3(256), 0, st, x, y, rows, mode, part)

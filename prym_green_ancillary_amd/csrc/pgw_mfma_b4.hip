@@ -11,4 +11,17 @@ void launch_dense_mfma_b4(int cfg, uint32_t nl, const DenseSweep& s, const MfmaT
     else dense_mfma_pick<4, 2>(cfg, s, t, lp, out, scale, m, st);
 }
 
+uint64_t mfma_group_prepare_b4(int cfg, uint32_t nl, const DenseSweep& s, const MfmaTable& t,
+                                const int8_t* lp, uint32_t* out, const uint32_t* scale, const ModP& m,
+                                MfmaSweepArgs& a) {
+    return nl == 4 ? mfma_group_prepare_t<4, 4>(cfg, s, t, lp, out, scale, m, a)
+                   : mfma_group_prepare_t<4, 2>(cfg, s, t, lp, out, scale, m, a);
+}
+
+void mfma_group_launch_b4(int cfg, uint32_t nl, bool fwd, const MfmaSweepArgs* jobs,
+                           const uint32_t* first, uint32_t njobs, uint64_t blocks, hipStream_t st) {
+    if (nl == 4) mfma_group_launch_t<4, 4>(cfg, fwd, jobs, first, njobs, blocks, st);
+    else mfma_group_launch_t<4, 2>(cfg, fwd, jobs, first, njobs, blocks, st);
+}
+
 }  // namespace pgw

@@ -101,12 +101,12 @@ std::vector<uint32_t> mfma_incidence_table(uint32_t n, uint32_t deg_out, bool fw
 // one thread per (panel v, 16-row group kg, column c): 16 words of a column in,
 // 16 digit bytes per limb and sign out (one 16-byte store each)
 template <int B, int NL>
-__global__ __launch_bounds__(256) void k_limb_split(const uint32_t* __restrict__ in,
-                                                    const uint32_t* __restrict__ scale,
-                                                    int8_t* __restrict__ lp, uint64_t v_count,
-                                                    uint32_t k_in, uint32_t k_stride, ModP m) {
+__device__ __forceinline__ void limb_split_one(const uint32_t* __restrict__ in,
+                                               const uint32_t* __restrict__ scale,
+                                               int8_t* __restrict__ lp, uint64_t v_count,
+                                               uint32_t k_in, uint32_t k_stride, const ModP& m,
+                                               uint64_t tid) {
     const uint32_t groups = k_stride / 16;
-    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     const uint64_t total = v_count * groups * B;
     if (tid >= total) return;
     const int c = (int)(tid % B);
@@ -151,6 +151,26 @@ __global__ __launch_bounds__(256) void k_limb_split(const uint32_t* __restrict__
     });
 }
 
+template <int B, int NL>
+__global__ __launch_bounds__(256) void k_limb_split(const uint32_t* __restrict__ in,
+                                                    const uint32_t* __restrict__ scale,
+                                                    int8_t* __restrict__ lp, uint64_t v_count,
+                                                    uint32_t k_in, uint32_t k_stride, ModP m) {
+    limb_split_one<B, NL>(in, scale, lp, v_count, k_in, k_stride, m,
+                          (uint64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+// many jobs' digit splits in one launch: job j's blocks [first[j], first[j+1])
+template <int B, int NL>
+__global__ __launch_bounds__(256) void k_limb_split_group(const LimbJob* __restrict__ jobs,
+                                                          const uint32_t* __restrict__ first,
+                                                          uint32_t njobs) {
+    const uint32_t j = group_job(first, njobs);
+    const LimbJob& J = jobs[j];
+    limb_split_one<B, NL>(J.in, J.scale, J.lp, J.v_count, J.k_in, J.k_stride, J.m,
+                          (uint64_t)(blockIdx.x - first[j]) * 256 + threadIdx.x);
+}
+
 static inline unsigned grid_for(uint64_t threads, unsigned block = 256) {
     uint64_t g = (threads + block - 1) / block;
     return (unsigned)std::max<uint64_t>(1, g);
@@ -174,6 +194,27 @@ void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, ui
     PGW_CHECK_LAUNCH("limb split");
 }
 
+uint64_t limb_group_blocks(const LimbJob& j, uint32_t b) {
+    return (j.v_count * (j.k_stride / 16) * b + 255) / 256;
+}
+
+void launch_limb_split_group(const LimbJob* jobs, const uint32_t* first, uint32_t njobs,
+                             uint64_t blocks, uint32_t b, uint32_t nl, hipStream_t st) {
+    if (blocks == 0) return;
+    require(blocks < (1ULL << 31), "grouped digit split grid too large");
+    const dim3 grid((unsigned)blocks), blk(256);
+#define PGW_LG(BB, NN) hipLaunchKernelGGL((k_limb_split_group<BB, NN>), grid, blk, 0, st, jobs, first, njobs)
+    if (nl == 2) {
+        if (b == 4) PGW_LG(4, 2); else if (b == 8) PGW_LG(8, 2); else if (b == 16) PGW_LG(16, 2);
+        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
+    } else {
+        if (b == 4) PGW_LG(4, 4); else if (b == 8) PGW_LG(8, 4); else if (b == 16) PGW_LG(16, 4);
+        else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
+    }
+#undef PGW_LG
+    PGW_CHECK_LAUNCH("grouped limb split");
+}
+
 // ---------------------------------------------------------------------------
 int mfma_config_count() { return 10; }
 
@@ -183,6 +224,34 @@ void launch_dense_mfma_b8(int, uint32_t, const DenseSweep&, const MfmaTable&, co
                           uint32_t*, const uint32_t*, const ModP&, hipStream_t);
 void launch_dense_mfma_b16(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
                            uint32_t*, const uint32_t*, const ModP&, hipStream_t);
+
+#define PGW_GROUP_DECL(BB)                                                                        \
+    uint64_t mfma_group_prepare_b##BB(int, uint32_t, const DenseSweep&, const MfmaTable&,          \
+                                      const int8_t*, uint32_t*, const uint32_t*, const ModP&,       \
+                                      MfmaSweepArgs&);                                              \
+    void mfma_group_launch_b##BB(int, uint32_t, bool, const MfmaSweepArgs*, const uint32_t*,        \
+                                 uint32_t, uint64_t, hipStream_t);
+PGW_GROUP_DECL(4)
+PGW_GROUP_DECL(8)
+PGW_GROUP_DECL(16)
+#undef PGW_GROUP_DECL
+
+uint64_t mfma_group_prepare(int cfg, uint32_t b, uint32_t nl, const DenseSweep& s, const MfmaTable& t,
+                            const int8_t* lp, uint32_t* out, const uint32_t* scale, const ModP& m,
+                            MfmaSweepArgs& a) {
+    if (b == 16) return mfma_group_prepare_b16(cfg, nl, s, t, lp, out, scale, m, a);
+    if (b == 8) return mfma_group_prepare_b8(cfg, nl, s, t, lp, out, scale, m, a);
+    if (b == 4) return mfma_group_prepare_b4(cfg, nl, s, t, lp, out, scale, m, a);
+    throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
+}
+
+void mfma_group_launch(int cfg, uint32_t b, uint32_t nl, bool fwd, const MfmaSweepArgs* jobs,
+                       const uint32_t* first, uint32_t njobs, uint64_t blocks, hipStream_t st) {
+    if (b == 16) mfma_group_launch_b16(cfg, nl, fwd, jobs, first, njobs, blocks, st);
+    else if (b == 8) mfma_group_launch_b8(cfg, nl, fwd, jobs, first, njobs, blocks, st);
+    else if (b == 4) mfma_group_launch_b4(cfg, nl, fwd, jobs, first, njobs, blocks, st);
+    else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
+}
 
 void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
                        const uint32_t* scale, uint32_t b, uint32_t nl, const ModP& m,

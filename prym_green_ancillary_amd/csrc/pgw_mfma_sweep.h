@@ -65,27 +65,6 @@ __host__ __device__ inline uint64_t digits_biased(uint32_t x, uint32_t p, const 
     return (uint64_t)(y + (int64_t)ds.bias);
 }
 
-struct ShiftWeights {
-    uint32_t w[8];
-};
-
-struct MfmaSweepArgs {
-    const uint32_t* inc;    // incidence table of the sweep direction (n + 1 words per output panel)
-    const int8_t* tabl;
-    const int8_t* lp;
-    const int8_t* zero;     // 1 KB of zero bytes (staging loads that have no data)
-    uint32_t n, deg_out;
-    uint64_t u_out, v_in;
-    uint32_t r_out, r_pad, k_in, k_pad;
-    uint32_t row_slices;
-    uint64_t col_groups;
-    uint32_t xcd_affinity;  // 1: grid is 8 * ceil(col_groups / (8 / row_slices)) blocks
-    uint32_t* out;
-    const uint32_t* scale;
-    ModP m;
-    ShiftWeights sw;
-};
-
 constexpr int kLdsStride = 64;  // bytes per staged row (one 64-wide K chunk)
 
 // Byte offset of (row, k) in a staged tile.  The four 16-byte pieces of a row
@@ -103,16 +82,17 @@ constexpr int kMaxN = 32;
 // column groups are spread over the 8/R XCDs sharing the slice.  The grid is
 // padded to a multiple of 8; returns false for the surplus blocks, which
 // leave before any barrier.
-__device__ __forceinline__ bool mfma_block_tile(const MfmaSweepArgs& a, uint64_t& cg, uint32_t& rs) {
+__device__ __forceinline__ bool mfma_block_tile(const MfmaSweepArgs& a, uint32_t block, uint64_t& cg,
+                                                uint32_t& rs) {
     if (a.xcd_affinity) {
-        const uint32_t xcd = blockIdx.x % 8, within = blockIdx.x / 8, G = 8 / a.row_slices;
+        const uint32_t xcd = block % 8, within = block / 8, G = 8 / a.row_slices;
         rs = xcd % a.row_slices;
         cg = (uint64_t)within * G + xcd / a.row_slices;
         return cg < a.col_groups;
     }
-    cg = blockIdx.x / a.row_slices;
-    rs = blockIdx.x % a.row_slices;
-    return true;
+    cg = block / a.row_slices;
+    rs = block % a.row_slices;
+    return cg < a.col_groups;
 }
 
 // Incidence codes of the workgroup's output panels, read from the sweep's
@@ -246,6 +226,7 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
 // ---------------------------------------------------------------------------
 template <int B, int NL, int RT, int CT, int WR, int WC, bool FWD>
 __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a) {
+    const uint32_t block = blockIdx.x;
     constexpr int NW = WR * WC;
     constexpr int NT = 64 * NW;
     constexpr int NS = 2 * NL - 1;
@@ -267,7 +248,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
 
     uint64_t cg;
     uint32_t rs;
-    if (!mfma_block_tile(a, cg, rs)) return;
+    if (!mfma_block_tile(a, block, cg, rs)) return;
     const int tid = threadIdx.x, wave = tid / 64;
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
@@ -533,6 +514,7 @@ constexpr int dma_waves_per_simd() {
 template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
 __global__ __launch_bounds__(64 * WR * WC, (dma_waves_per_simd<B, NL, RT, CT, WR, WC, STG>()))
 void k_dense_mfma_dma(MfmaSweepArgs a) {
+    const uint32_t block = blockIdx.x;
     constexpr int NW = WR * WC;
     constexpr int NT = 64 * NW;
     constexpr int NS = 2 * NL - 1;
@@ -557,7 +539,7 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
 
     uint64_t cg;
     uint32_t rs;
-    if (!mfma_block_tile(a, cg, rs)) return;
+    if (!mfma_block_tile(a, block, cg, rs)) return;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
@@ -767,7 +749,7 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
 // of c+2, so at chunk c+1 it waits for all but its last PER loads.
 // ---------------------------------------------------------------------------
 template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
-__global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepArgs a) {
+__device__ __forceinline__ void dense_mfma_areg_body(const MfmaSweepArgs& a, uint32_t block) {
     constexpr int NW = WR * WC;
     constexpr int NT = 64 * NW;
     constexpr int NS = 2 * NL - 1;
@@ -791,7 +773,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
 
     uint64_t cg;
     uint32_t rs;
-    if (!mfma_block_tile(a, cg, rs)) return;
+    if (!mfma_block_tile(a, block, cg, rs)) return;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int wr = wave / WC, wc = wave % WC;
     const uint32_t wg_row0 = rs * ROWS;
@@ -1014,17 +996,30 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
     mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc, sc);
 }
 
-// kernel variants of a tile configuration: DMA 0 register-staged, 2 or 3
-// direct-to-LDS stages, 4 + s table fragments in registers and s digit-plane
-// stages
-template <int B, int NL, int RT, int CT, int WR, int WC, int DMA = 0>
-static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
-                           const uint32_t* scale, const ModP& m, hipStream_t st) {
+template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
+__global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepArgs a) {
+    dense_mfma_areg_body<B, NL, RT, CT, WR, WC, STG, FWD>(a, blockIdx.x);
+}
+
+// many sectors' sweeps in one launch (same B, NL, tile and direction): the
+// arguments of job j at jobs[j], its blocks [first[j], first[j+1])
+template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
+__global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg_group(
+    const MfmaSweepArgs* __restrict__ jobs, const uint32_t* __restrict__ first, uint32_t njobs) {
+    const uint32_t j = group_job(first, njobs);
+    const MfmaSweepArgs a = jobs[j];
+    dense_mfma_areg_body<B, NL, RT, CT, WR, WC, STG, FWD>(a, blockIdx.x - first[j]);
+}
+
+// Arguments and grid of one sweep launch of tile configuration (RT, CT, WR, WC).
+template <int B, int RT, int CT, int WR, int WC>
+static uint64_t mfma_args_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
+                              const uint32_t* scale, const ModP& m, MfmaSweepArgs& a) {
     constexpr int SLOTS = WC * CT * (16 / B);
     constexpr int ROWS = WR * RT * 16;
     require(s.n <= (uint32_t)kMaxN, "MFMA sweep supports n <= 32");
     require(s.k < 16u, "MFMA sweep supports k < 16");
-    MfmaSweepArgs a;
+    a = MfmaSweepArgs{};
     a.inc = s.inc;
     require(a.inc != nullptr, "MFMA sweep needs the operator's incidence table");
     a.tabl = t.d;
@@ -1054,6 +1049,17 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     const uint64_t blocks = a.xcd_affinity ? 8 * ((a.col_groups + G - 1) / G)
                                            : a.col_groups * a.row_slices;
     require(blocks < (1ULL << 31), "MFMA sweep grid too large");
+    return blocks;
+}
+
+// kernel variants of a tile configuration: DMA 0 register-staged, 2 or 3
+// direct-to-LDS stages, 4 + s table fragments in registers and s digit-plane
+// stages
+template <int B, int NL, int RT, int CT, int WR, int WC, int DMA = 0>
+static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,
+                           const uint32_t* scale, const ModP& m, hipStream_t st) {
+    MfmaSweepArgs a;
+    const uint64_t blocks = mfma_args_cfg<B, RT, CT, WR, WC>(s, t, lp, out, scale, m, a);
 #define PGW_MF(K, ...) hipLaunchKernelGGL((K<B, NL, RT, CT, WR, WC, __VA_ARGS__>), dim3((unsigned)blocks), \
                                           dim3(64 * WR * WC), 0, st, a)
     if constexpr (DMA >= 4) {
@@ -1065,6 +1071,39 @@ static void dense_mfma_cfg(const DenseSweep& s, const MfmaTable& t, const int8_t
     }
 #undef PGW_MF
     PGW_CHECK_LAUNCH("MFMA sweep");
+}
+
+// Grouped sweeps use the register-A configurations (7, 8, 9).  prepare: the
+// job's arguments and its block count rounded up to a multiple of 8.
+template <int B, int NL>
+static uint64_t mfma_group_prepare_t(int cfg, const DenseSweep& s, const MfmaTable& t, const int8_t* lp,
+                                     uint32_t* out, const uint32_t* scale, const ModP& m, MfmaSweepArgs& a) {
+    constexpr int CT = NL == 4 ? 2 : 4;
+    uint64_t blocks = 0;
+    switch (cfg) {
+        case 7: blocks = mfma_args_cfg<B, 2, CT, 8, 1>(s, t, lp, out, scale, m, a); break;
+        case 8: case 9: blocks = mfma_args_cfg<B, 2, CT, 4, 1>(s, t, lp, out, scale, m, a); break;
+        default: throw Error(PGW_ERR_UNSUPPORTED, "grouped MFMA sweeps use tile configurations 7-9");
+    }
+    return (blocks + 7) / 8 * 8;
+}
+
+template <int B, int NL>
+static void mfma_group_launch_t(int cfg, bool fwd, const MfmaSweepArgs* jobs, const uint32_t* first,
+                                uint32_t njobs, uint64_t blocks, hipStream_t st) {
+    constexpr int CT = NL == 4 ? 2 : 4;
+    require(blocks < (1ULL << 31), "grouped MFMA sweep grid too large");
+#define PGW_MG(WR, STG, F)                                                                          \
+    hipLaunchKernelGGL((k_dense_mfma_areg_group<B, NL, 2, CT, WR, 1, STG, F>), dim3((unsigned)blocks), \
+                       dim3(64 * WR), 0, st, jobs, first, njobs)
+    switch (cfg) {
+        case 7: if (fwd) PGW_MG(8, 2, true); else PGW_MG(8, 2, false); break;
+        case 8: if (fwd) PGW_MG(4, 2, true); else PGW_MG(4, 2, false); break;
+        case 9: if (fwd) PGW_MG(4, 3, true); else PGW_MG(4, 3, false); break;
+        default: throw Error(PGW_ERR_UNSUPPORTED, "grouped MFMA sweeps use tile configurations 7-9");
+    }
+#undef PGW_MG
+    PGW_CHECK_LAUNCH("grouped MFMA sweep");
 }
 
 template <int B, int NL>

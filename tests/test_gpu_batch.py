@@ -45,17 +45,13 @@ def make_op(ctx, job, method, seed=3):
     return op, (T, E, dr, dc, v, pairs)
 
 
-@pytest.mark.parametrize("use_graph,balance", [
-    (True, False), (False, False),
-    pytest.param(True, True, marks=pytest.mark.skip(
-        reason="aborted once on MI355X inside pgw_batch_step (SIGABRT, message lost to pytest's "
-               "fd capture, no GPU fault named); no cause shown yet, so this case stays off the "
-               "GPU. For this mix every balanced quota is 1, so its graphs equal (True, False)'s")),
-    (False, True)])
-def test_batch_uneven_sectors_match_single_runs_and_oracle(ctx, use_graph, balance):
+@pytest.mark.parametrize("use_graph,grouped", [(True, True), (True, False), (False, True),
+                                              (False, False)])
+def test_batch_rounds_match_single_runs_and_oracle(ctx, use_graph, grouped):
+    # MFMA jobs of one block width share launches (grouped); plain jobs run alone
     jobs = small_mix()
     lengths = [5 + 7 * i for i in range(len(jobs))]  # uneven lengths: jobs finish at different steps
-    batch = pg.Batch(ctx, nstreams=4, use_graph=use_graph, balance=balance)
+    batch = pg.Batch(ctx, use_graph=use_graph, grouped=grouped)
     ops, ids, state = [], [], []
     for j, L in zip(jobs, lengths):
         method = pg.METHOD_MFMA if j.index % 2 == 0 else pg.METHOD_PLAIN
@@ -65,9 +61,12 @@ def test_batch_uneven_sectors_match_single_runs_and_oracle(ctx, use_graph, balan
         ids.append(batch.add(op, st[4], L))
         ops.append(op)
         state.append(st)
-    steps = 0
+    steps, seen_groups = 0, 0
     while batch.step(3):
         steps += 1
+        seen_groups = max(seen_groups, batch.groups()[1])
+    if grouped:
+        assert seen_groups >= 2  # the MFMA jobs of one block width ran as a group
     got = {i: batch.result(i) for i in ids}
     if use_graph:
         assert batch.graph_builds() >= 2  # the active set shrank at least once
@@ -84,14 +83,13 @@ def test_batch_uneven_sectors_match_single_runs_and_oracle(ctx, use_graph, balan
 
 
 @pytest.mark.skip(reason="aborted once on MI355X with glibc 'free(): invalid next size (fast)' in "
-                         "pgw_batch_destroy after this test's batch run (host heap corruption, "
-                         "detected at the first free after it happened); the library's host "
-                         "paths it uses (against a stand-in HIP runtime) and the oracle pass "
-                         "AddressSanitizer and ThreadSanitizer on the CPU, no cause shown yet, "
-                         "so the case stays off the GPU")
+                         "pgw_batch_destroy after this test's batch run under the earlier "
+                         "multi-stream graph engine (host heap corruption, detected at the first "
+                         "free after it happened); no cause has been shown, so the case stays off "
+                         "the GPU")
 def test_batch_mixes_cyclic_and_synthetic(ctx, g12):
     p, b = g12["modulus"], 4
-    batch = pg.Batch(ctx, nstreams=3)
+    batch = pg.Batch(ctx)
     entries = []
     for sector in range(g12["cyclic"]["order"]):
         o = oc.cyclic_op(g12, sector)
@@ -116,6 +114,44 @@ def test_batch_mixes_cyclic_and_synthetic(ctx, g12):
             assert (seq == oseq).all() and (curv == ocurv).all()
 
 
+def test_batch_c3_mix_rounds_match_lone_runs_and_oracle(ctx):
+    """The batching workload itself: 64 uneven sectors (seed 3: n 12..22,
+    k 3..6, M 20..300, p = 65521, b 8 or 16, N 14300..4.79M), two rounds
+    each, grouped and graph-replayed; every job equals its lone run, the
+    smallest sectors equal the oracle."""
+    jobs = wl.uneven_sectors(seed=3, count=64)
+    max_len = 9  # two rounds
+    batch = pg.Batch(ctx, use_graph=True, grouped=True)
+    entries = []
+    for j in jobs:
+        dr, dc, v, pairs = wl.sector_state(j, 3)
+        T, E = j.spec.tables()
+        op = pg.Operator.synthetic(ctx, j.spec, j.block, dr, dc, 2, pairs, tables=(T, E))
+        entries.append((j, op, batch.add(op, v, max_len), (T, E, dr, dc, v, pairs)))
+    _log(f"c3 mix: {len(entries)} sectors added")
+    while batch.step(4):
+        pass
+    groups, grouped = batch.groups()
+    res = {jid: batch.result(jid) for _, _, jid, _ in entries}
+    batch.close()
+    _log(f"c3 mix: done ({groups} groups, {grouped} grouped jobs in the last plan)")
+    small = sorted(entries, key=lambda e: e[0].cols)[:3]
+    for j, op, jid, st in entries:
+        T, E, dr, dc, v, pairs = st
+        seq, curv = res[jid]
+        lone_seq, lone_curv = op.sequence(v, max_len)
+        assert (seq == lone_seq).all() and (curv == lone_curv).all(), j.index
+        op.close()
+    for j, op, jid, st in small:
+        T, E, dr, dc, v, pairs = st
+        _log(f"c3 oracle check: sector {j.index} N={j.cols}")
+        o = oc.synthetic_op(j.spec.n, j.spec.k, j.spec.m_src, j.spec.m_tgt, j.spec.prime, T, E,
+                            dr, dc, 2, pairs.reshape(-1))
+        oseq, ocurv = o.sequence(v, j.block, max_len)
+        seq, curv = res[jid]
+        assert (seq == oseq).all() and (curv == ocurv).all(), j.index
+
+
 def test_batch_rejects_second_job_on_one_operator(ctx):
     j = small_mix(count=1)[0]
     op, st = make_op(ctx, j, pg.METHOD_PLAIN)
@@ -128,10 +164,9 @@ def test_batch_rejects_second_job_on_one_operator(ctx):
 
 
 @pytest.mark.skip(reason="aborted once on MI355X with glibc 'corrupted size vs. prev_size' in a "
-                         "server thread while the test waited on a ticket (host heap corruption); "
-                         "the server's host paths pass AddressSanitizer and ThreadSanitizer on "
-                         "the CPU against a stand-in HIP runtime, no cause shown yet, so the "
-                         "case stays off the GPU")
+                         "server thread while the test waited on a ticket, under the earlier "
+                         "multi-stream graph engine (host heap corruption); no cause has been "
+                         "shown, so the case stays off the GPU")
 def test_server_results_identical_alone_and_in_mix(ctx):
     jobs = small_mix(count=8, seed=7)
     lengths = [9 + 5 * (i % 4) for i in range(len(jobs))]
@@ -165,6 +200,48 @@ def test_server_results_identical_alone_and_in_mix(ctx):
         lone_seq, lone_curv = op.sequence(v, L)
         assert (seq == lone_seq).all() and (curv == lone_curv).all(), j.index
         assert list(times) == sorted(times)
+
+
+def test_server_c5_stream_alone_equals_mix(ctx):
+    """The serving workload itself: the first 12 jobs of the seed-5 Poisson
+    stream (N 22264..18.2M, p in {65521, 2^31-1}, b in {8, 16}), two rounds
+    each, through a server with 8 slots and through one with a single slot;
+    every job's results agree, the three smallest equal the oracle."""
+    jobs = wl.poisson_stream(seed=5, count=12)
+    max_len = 9
+    prepared = []
+    for j in jobs:
+        dr, dc, v, pairs = wl.sector_state(j, 5)
+        T, E = j.spec.tables()
+        desc = pg.Operator.synthetic_desc(j.spec, j.block, pg.METHOD_AUTO, T, E)
+        prepared.append((j, desc, (T, E), dr, dc, v, pairs))
+    _log(f"c5 stream: {len(prepared)} jobs prepared")
+
+    def serve(max_active):
+        srv = pg.Server(ctx, 96 << 30, max_active=max_active, replays_per_step=2)
+        tickets = [srv.submit(desc, dr, dc, v, max_len, 2, pairs, keep=keep)
+                   for j, desc, keep, dr, dc, v, pairs in prepared]
+        outs = []
+        for t in tickets:
+            assert srv.wait(t, 600.0) == pg.Server.DONE
+            outs.append(srv.result(t))
+        stats = srv.stats()
+        srv.close()
+        _log(f"c5 stream: max_active={max_active} stats {stats}")
+        return outs, stats
+
+    mixed, stats = serve(8)
+    alone, _ = serve(1)
+    assert stats["completed"] == len(prepared) and stats["peak_active"] >= 2
+    for (j, *_), a, b1 in zip(prepared, mixed, alone):
+        assert (a[0] == b1[0]).all() and (a[1] == b1[1]).all(), j.index
+    for q in sorted(range(len(prepared)), key=lambda q: prepared[q][0].cols)[:3]:
+        j, desc, (T, E), dr, dc, v, pairs = prepared[q]
+        _log(f"c5 oracle check: job {j.index} N={j.cols} p={j.spec.prime} b={j.block}")
+        o = oc.synthetic_op(j.spec.n, j.spec.k, j.spec.m_src, j.spec.m_tgt, j.spec.prime, T, E,
+                            dr, dc, 2, pairs.reshape(-1))
+        oseq, ocurv = o.sequence(v, j.block, max_len)
+        assert (mixed[q][0] == oseq).all() and (mixed[q][1] == ocurv).all(), j.index
 
 
 def test_server_rank_stage_matches_standalone_consumer(ctx, g12, g12_reference):
