@@ -11,7 +11,9 @@ be bit-identical before any number is printed.
 Other workloads (`--workload`): `sector` (the mid sector n=20, k=5, M=200,
 M'=220, N = 3,100,800) the same way; `batch` (64 uneven sectors, batched
 against one at a time); `serve` (a Poisson stream of sector jobs through the
-serving loop, p50/p99 latency).
+serving loop, p50/p99 latency); `paper` (every base sector of the reference's
+genus-20 or genus-22 instance, median sequence-loop time beside the published
+one).
 
 The JSON line carries `roofline` (the MFMA sweep against dense int8 peak;
 `traffic` from the FETCH_SIZE/WRITE_SIZE counters committed in
@@ -220,8 +222,10 @@ def bench_batch(args):
 
 def bench_serve(args):
     """Poisson stream (seed 5, 2 jobs/s) replayed in real time into the
-    serving loop; each job runs min(2N/b + 8, --max-len) sequence terms and,
-    unless --no-rank, the host rank stage on its sequence.  Reports
+    serving loop; each job runs its sequence terms (to the full length 2N/b
+    + 8 when that is at most --full-max, so that its rank stage can finish;
+    the others stop at --max-len) and, unless --no-rank, the host rank stage
+    on its sequence.  Reports
     submit-to-finish latency p50/p99 and throughput, against the same stream
     through a server that runs one job at a time."""
     import prym_green_ancillary_amd as pg
@@ -236,7 +240,8 @@ def bench_serve(args):
         dr, dc, v, pairs = wl.sector_state(j, 5)
         T, E = j.spec.tables()
         desc = pg.Operator.synthetic_desc(j.spec, j.block, pg.METHOD_AUTO, T, E)
-        L = min(j.full_length(), args.max_len)
+        full = j.full_length()
+        L = full if full <= args.full_max else min(full, args.max_len)
         prepared.append((j, desc, (T, E), dr, dc, v, pairs, L))
     t_setup = time.perf_counter() - t_setup
     budget = int(args.hbm_fraction * total)
@@ -279,6 +284,14 @@ def bench_serve(args):
             sys.exit(1)
     applies = sum(2 * ((L - 1 + 3) // 4) for (_, _, _, _, _, _, _, L) in prepared)
     Ns = [j.cols for j in jobs]
+    is_full = [L >= j.full_length() for (j, _, _, _, _, _, _, L) in prepared]
+
+    def rank_summary(sel):
+        st = [r[3].status for r, f in zip(mixed, is_full) if f == sel]
+        ranks = [int(r[3].rank) for r, f in zip(mixed, is_full) if f == sel]
+        return {"jobs": len(st),
+                "status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
+                "ranks": ranks if sel else None}
     line = {
         "metric": "sector job latency p50/p99 + throughput under a Poisson stream; results "
                   "identical alone vs in the mix",
@@ -290,7 +303,9 @@ def bench_serve(args):
         "dtype": "int8-limb/int32 (u32 residues mod 65521 or 2^31-1)", "data": "synthetic",
         "config": {"workload": f"Poisson stream seed 5, 2 jobs/s, {len(jobs)} jobs, N log-uniform "
                                f"1e4..3e7 ({min(Ns)}..{max(Ns)}), p in {{65521, 2^31-1}}, "
-                               f"b in {{8,16}}, terms min(2N/b+8, {args.max_len})",
+                               f"b in {{8,16}}, terms 2N/b+8 when that is at most {args.full_max}, "
+                               f"else {args.max_len}",
+                   "full_length_jobs": int(sum(is_full)),
                    "jobs": len(jobs), "hbm_budget_bytes": budget, "max_active": args.max_active,
                    "parallelism": "single GPU, continuous batching"},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
@@ -308,10 +323,92 @@ def bench_serve(args):
             "threads_per_job": args.rank_threads,
             "mean_s": round(float(np.mean([r[4] for r in mixed])), 4),
             "max_s": round(float(np.max([r[4] for r in mixed])), 4),
-            "status_counts": {str(k): int(v) for k, v in zip(*np.unique(
-                [r[3].status for r in mixed], return_counts=True))},
+            "status_legend": {"0": "running (needs more terms)", "1": "success: rank final",
+                              "2": "exceeds dimension", "3": "sequence limit"},
+            "full_length": rank_summary(True),
+            "capped": rank_summary(False),
         },
         "setup_s": round(t_setup, 2),
+        "device": dev_name,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_paper(args):
+    """The reference's own instances (tests/golden/g<genus>.json) with its
+    paper recipe: b = 4, Dr/Dc/V from seed 1, two row-mixing rounds from seed
+    10001, 2N/4 + 50 terms.  Every sector's sequence loop (pgw_sequence_run:
+    V upload, the captured round replayed, sequence and final block back to
+    the host) is timed on the wall clock; the median over the sectors is the
+    quantity the reference publishes for these instances (median base-sector
+    sequence-loop time, RESULTS.md:26-31; other hardware, same recipe).
+
+    --eager-rounds R instead runs R eager rounds (apply, Gram projection)
+    on the first sector, without graphs, for kernel traces."""
+    import prym_green_ancillary_amd as pg
+
+    gold = ROOT / "tests" / "golden"
+    inst = json.loads((gold / f"g{args.genus}.json").read_text())
+    case = next(c for c in json.loads((gold / "cases.json").read_text())["cases"]
+                if c["genus"] == args.genus)
+    p, b = inst["modulus"], 4
+    sectors = (list(range(case["order"])) if not args.paper_sectors
+               else [int(x) for x in args.paper_sectors.split(",")])
+    ctx = pg.Context(0)
+    dev_name, _ = ctx.device_info()
+    loops, per_sector = [], []
+    for s in sectors:
+        d, keep = pg.Operator.cyclic_desc(inst, s, b)
+        rows, cols = pg.Operator.dims(d)
+        dr, dc, v = pg.reference_random_state(1, s, p, rows, cols, b)
+        pairs = pg.rowmix_plan(pg.rowmix_realized_seed(10001, "eliminated", s, rows, p), rows, 2, p)
+        op = pg.Operator.cyclic(ctx, inst, s, b, dr, dc, 2, pairs)
+        if args.eager_rounds:
+            import torch
+            x = torch.from_numpy(v.view(np.int32).copy()).cuda()
+            y = torch.empty_like(x)
+            g = torch.zeros(b * (b + 1) // 2, dtype=torch.int32, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.eager_rounds):
+                op.apply_square(x.data_ptr(), y.data_ptr())
+                op.gram_upper(x.data_ptr(), y.data_ptr(), g.data_ptr())
+                x, y = y, x
+            ctx.synchronize()
+            us = (time.perf_counter() - t0) / args.eager_rounds * 1e6
+            print(json.dumps({"eager": f"g{args.genus} sector {s}", "rows": rows, "cols": cols,
+                              "rounds": args.eager_rounds, "us_per_apply_and_gram": round(us, 2)}),
+                  flush=True)
+            op.close()
+            return
+        max_len = 2 * cols // b + 50
+        t0 = time.perf_counter()
+        seq, _ = op.sequence(v, max_len)
+        t = time.perf_counter() - t0
+        op.close()
+        applies = (seq.shape[0] - 1) // 2
+        loops.append(t)
+        per_sector.append({"sector": s, "rows": rows, "cols": cols, "terms": int(seq.shape[0]),
+                           "loop_s": round(t, 4), "block_applies_per_s": round(applies / t, 1)})
+        print(f"g{args.genus} sector {s}: {rows} x {cols}, {seq.shape[0]} terms in {t:.3f} s",
+              file=sys.stderr, flush=True)
+    med = float(np.median(loops))
+    pub = case["published_base_gpu_time_median_seconds"]
+    line = {
+        "metric": f"median base-sector sequence-loop time, genus {args.genus} (the reference's "
+                  "published metric for its paper instances)",
+        "value": round(med, 4), "unit": "s", "n_gpus": 1, "steps": len(sectors), "warmup": 0,
+        "ms_per_step": round(1000.0 * med, 2), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": round(med / pub, 4),
+        "dtype": f"u32 residues mod {p} (exact u64 sums)", "data": f"data/g{args.genus}.json of the reference",
+        "config": {"workload": f"g{args.genus} eliminated base sectors {sectors[0]}..{sectors[-1]}, "
+                               f"b=4, seed 1, row mixing 2 rounds seed 10001, 2N/4+50 terms",
+                   "sectors": len(sectors), "parallelism": "single GPU, one sector at a time"},
+        "published": {"median_loop_s": pub, "gpu": case["published_gpu_model"],
+                      "speedup_vs_published": round(pub / med, 3)},
+        "median_block_applies_per_s": round(float(np.median([x["block_applies_per_s"] for x in per_sector])), 1),
+        "per_sector": per_sector,
         "device": dev_name,
         "roofline": None, "cpu_baseline": None,
     }
@@ -326,13 +423,21 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", choices=["sector", "large", "batch", "serve"], default="large",
+    ap.add_argument("--workload", choices=["sector", "large", "batch", "serve", "paper"], default="large",
                     help="large: the N=53.8M sector (default); sector: the mid-size sector; "
-                         "batch: 64 uneven sectors; serve: Poisson job stream")
+                         "batch: 64 uneven sectors; serve: Poisson job stream; paper: the "
+                         "reference's g20/g22 instances")
+    ap.add_argument("--genus", type=int, default=22, choices=[20, 22, 24], help="paper: instance")
+    ap.add_argument("--paper-sectors", default="", help="paper: comma-separated sectors (default all)")
+    ap.add_argument("--eager-rounds", type=int, default=0,
+                    help="paper: time this many eager apply+Gram rounds on the first sector instead")
     ap.add_argument("--rounds", type=int, default=4, help="batch: sequence rounds per sector")
     ap.add_argument("--sectors", type=int, default=64, help="batch: sectors in the mix")
     ap.add_argument("--jobs", type=int, default=24, help="serve: jobs in the stream")
-    ap.add_argument("--max-len", type=int, default=33, help="serve: sequence-term cap per job")
+    ap.add_argument("--max-len", type=int, default=33,
+                    help="serve: sequence-term cap of the jobs that do not run to full length")
+    ap.add_argument("--full-max", type=int, default=16384,
+                    help="serve: jobs whose full length 2N/b+8 is at most this run to full length")
     ap.add_argument("--max-active", type=int, default=16, help="serve: concurrent jobs")
     ap.add_argument("--hbm-fraction", type=float, default=0.5, help="serve: HBM budget share")
     ap.add_argument("--no-rank", action="store_true", help="serve: skip the host rank stage")
@@ -346,6 +451,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "paper":
+        return bench_paper(args)
 
     import torch
     import prym_green_ancillary_amd as pg

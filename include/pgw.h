@@ -197,6 +197,24 @@ int pgw_gram_upper_noalloc(pgw_ctx* ctx, uint32_t prime, uint32_t block, uint64_
                            const uint32_t* d_x, const uint32_t* d_y, uint32_t* d_out,
                            void* stream);
 
+/* The exact host apply of a cyclic operator (eliminated or a deformation
+ * variant): h_out = S h_in, S = Dc F^T P^T Dr P F Dc, on a cols x width host
+ * block, with the same description, preconditioners and row-mixing plan that
+ * pgw_operator_create takes.  Replaces cpu_apply_eliminated_square
+ * (cuda/cuprym_cyclic.cu:1591-1966): matrix-free from the sector layout in
+ * the reference's step order, on the host only (no device is touched).  The
+ * CLI's --validate-rowmix compares the device apply with it entry by entry
+ * (validate_eliminated_cpu_cuda, cuda/cuprym_cyclic.cu:1970-2132). */
+int pgw_cyclic_apply_cpu(const pgw_operator_desc* desc, const uint32_t* h_row_precond,
+                         const uint32_t* h_col_precond, uint32_t rowmix_rounds,
+                         const pgw_rowmix_pair* h_pairs, const uint32_t* h_in, uint32_t width,
+                         uint32_t* h_out);
+/* deterministic_dense_block (cuda/cuprym_cyclic.cu:1410-1420): rows x cols
+ * residues mod prime, row-major, from the splitmix64 stream seeded with
+ * salt ^ (0xD6E8FEB86659FD93 + rows * 1315423911 + cols * 2654435761). */
+int pgw_deterministic_block(uint64_t rows, uint64_t cols, uint32_t prime, uint64_t salt,
+                            uint32_t* h_out);
+
 /* Progress callback: terms done, terms total. */
 typedef void (*pgw_progress_fn)(uint64_t done, uint64_t total, void* user);
 

@@ -11,7 +11,6 @@ from __future__ import annotations
 
 import ctypes
 import json
-import os
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -20,6 +19,7 @@ import numpy as np
 __all__ = [
     "NativeLibraryError", "PgwError", "lib", "library_path", "Context", "Operator",
     "reference_random_state", "rowmix_realized_seed", "rowmix_plan", "rowmix_plan_hash",
+    "deterministic_block", "cyclic_apply_cpu",
     "fnv1a64_u32", "synthetic_table", "SyntheticSpec", "synthetic_spec_from_json",
     "load_cyclic_instance", "METHOD_AUTO", "METHOD_PLAIN", "METHOD_MFMA", "sequence_length",
     "RankComputation", "RANK_RUNNING", "RANK_SUCCESS", "RANK_EXCEEDS_DIMENSION",
@@ -105,19 +105,30 @@ EXPORTED = [
     "pgw_server_destroy", "pgw_job_footprint", "pgw_rank_create", "pgw_rank_step",
     "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
     "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_bench_stages",
-    "pgw_gram_upper_noalloc",
+    "pgw_gram_upper_noalloc", "pgw_cyclic_apply_cpu", "pgw_deterministic_block",
 ]
 
 
 def library_path() -> Path:
-    return Path(os.environ.get("PGW_LIBRARY", LIB_PATH))
+    """The in-tree libpgw.so; no other build is ever loaded."""
+    return LIB_PATH
 
 
 def lib():
-    """Load libpgw.so (built in-tree by __graft_entry__.build())."""
+    """Load libpgw.so (built in-tree by __graft_entry__.build()).
+
+    PyTorch is imported first when it is installed: its wheel carries its own
+    HIP and HSA runtimes, which libpgw then shares (same soname).  Loaded
+    the other way round, libpgw would pull in /opt/rocm's runtimes and a
+    later `import torch` a second pair beside them, two runtimes driving one
+    GPU in one process."""
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        import torch  # noqa: F401  (runtime load order, see above)
+    except ImportError:
+        pass
     path = library_path()
     if not path.exists():
         raise NativeLibraryError(f"{path} is missing: build it with __graft_entry__.build() "
@@ -149,6 +160,8 @@ def lib():
         "pgw_apply_forward": (I, [vp, vp, vp, vp]),
         "pgw_gram_upper": (I, [vp, vp, vp, vp, vp]),
         "pgw_gram_upper_noalloc": (I, [vp, u32, u32, u64, vp, vp, vp, vp]),
+        "pgw_cyclic_apply_cpu": (I, [ctypes.POINTER(OperatorDesc), vp, vp, u32, vp, vp, u32, vp]),
+        "pgw_deterministic_block": (I, [u64, u64, u32, u64, vp]),
         "pgw_sequence_run": (I, [vp, vp, u64, vp, vp, ctypes.POINTER(u64), PROGRESS_FN, vp]),
         "pgw_kernel_vectors": (I, [vp, vp, vp, u64, vp, vp]),
         "pgw_bench_apply": (I, [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_double),
@@ -216,6 +229,25 @@ def reference_random_state(seed: int, sector: int, p: int, rows: int, cols: int,
     v = np.empty(cols * b, np.uint32)
     _check(lib().pgw_reference_random_state(seed, sector, p, rows, cols, b, _ptr(dr), _ptr(dc), _ptr(v)))
     return dr, dc, v
+
+
+def deterministic_block(rows: int, cols: int, p: int, salt: int) -> np.ndarray:
+    """deterministic_dense_block (cuprym_cyclic.cu:1410-1420), rows x cols row-major."""
+    out = np.empty(max(1, rows * cols), np.uint32)
+    _check(lib().pgw_deterministic_block(rows, cols, p, salt, _ptr(out)))
+    return out[: rows * cols]
+
+
+def cyclic_apply_cpu(desc, dr, dc, rounds: int, pairs, x, width: int) -> np.ndarray:
+    """S x on the host for a cyclic operator description (pgw_cyclic_apply_cpu:
+    the reference's cpu_apply_eliminated_square, matrix-free; no device)."""
+    x = _u32(x).reshape(-1)
+    dr, dc = _u32(dr).reshape(-1), _u32(dc).reshape(-1)
+    pr = _u32(pairs).reshape(-1) if pairs is not None and len(pairs) else np.zeros(4, np.uint32)
+    out = np.empty(max(1, x.size), np.uint32)
+    _check(lib().pgw_cyclic_apply_cpu(ctypes.byref(desc), _ptr(dr), _ptr(dc), rounds, _ptr(pr),
+                                      _ptr(x), width, _ptr(out)))
+    return out[: x.size]
 
 
 def rowmix_realized_seed(seed: int, op_name: str, sector: int, rows: int, p: int) -> int:

@@ -17,7 +17,6 @@
 #include <filesystem>
 #include <iomanip>
 #include <iostream>
-#include <random>
 #include <vector>
 
 #include "cli_common.h"
@@ -196,32 +195,57 @@ int main(int argc, char** argv) {
                   << " MiB, method=" << (info.method == PGW_METHOD_MFMA ? "mfma" : "plain") << std::endl;
 
         if (has(a, "--validate-rowmix")) {
-            // rowmix transpose identity on the host and symmetry of S on the GPU
-            std::mt19937_64 g(0xA11CE001ULL);
+            // validate_eliminated_cpu_cuda (cuda/cuprym_cyclic.cu:1970-2132): the
+            // row-mixing transpose identity on two deterministic blocks, then S X
+            // on the device against the exact host apply on three salted blocks,
+            // entry by entry; the symmetry <S u, w> = <u, S w> as an extra check
+            const uint32_t p = inst.modulus;
             std::vector<uint32_t> x(rows * num_v), y(rows * num_v);
-            for (auto& e : x) e = (uint32_t)(g() % inst.modulus);
-            for (auto& e : y) e = (uint32_t)(g() % inst.modulus);
+            check(pgw_deterministic_block(rows, num_v, p, 0xA11CE001ULL, x.data()));
+            check(pgw_deterministic_block(rows, num_v, p, 0xB0B5EED2ULL, y.data()));
             std::vector<uint32_t> px = x, pty = y;
-            host_rowmix(px, pairs, pc, rounds, num_v, inst.modulus, false);
-            host_rowmix(pty, pairs, pc, rounds, num_v, inst.modulus, true);
-            if (dot_mod(px, y, inst.modulus) != dot_mod(x, pty, inst.modulus))
-                throw std::runtime_error("rowmix transpose identity failed");
-            std::cout << "Validation matched: rowmix transpose identity" << std::endl;
-            for (uint64_t salt : {0x404ULL, 0x505ULL, 0x606ULL}) {
-                std::mt19937_64 h(salt);
-                std::vector<uint32_t> u(cols * num_v), w(cols * num_v), su(u.size()), sw(w.size());
-                for (auto& e : u) e = (uint32_t)(h() % inst.modulus);
-                for (auto& e : w) e = (uint32_t)(h() % inst.modulus);
-                // host-buffer block-apply: the recovery entry with generators
-                // (G_0, G_1) = (0, I) returns D = S u
-                std::vector<uint32_t> prod(rows * num_v), gens2(2 * num_v * num_v, 0);
-                for (uint32_t c = 0; c < num_v; ++c) gens2[num_v * num_v + c * num_v + c] = 1;
-                check(pgw_kernel_vectors(op, u.data(), gens2.data(), 2, su.data(), prod.data()));
-                check(pgw_kernel_vectors(op, w.data(), gens2.data(), 2, sw.data(), prod.data()));
-                if (dot_mod(su, w, inst.modulus) != dot_mod(u, sw, inst.modulus))
-                    throw std::runtime_error("GPU square operator is not symmetric on salt " + std::to_string(salt));
+            host_rowmix(px, pairs, pc, rounds, num_v, p, false);
+            host_rowmix(pty, pairs, pc, rounds, num_v, p, true);
+            const uint64_t left = dot_mod(px, y, p), right = dot_mod(x, pty, p);
+            if (left != right)
+                throw std::runtime_error(op_name + " target rowmix transpose identity failed: <Tx,y>=" +
+                                         std::to_string(left) + ", <x,T^t y>=" + std::to_string(right));
+            std::cout << "Validation matched: " << op_name << " target rowmix transpose identity" << std::endl;
+            // host-buffer block-apply on the device: the recovery entry with
+            // generators (G_0, G_1) = (0, I) returns D = S u
+            std::vector<uint32_t> gens2(2 * num_v * num_v, 0), prod(rows * num_v);
+            for (uint32_t c = 0; c < num_v; ++c) gens2[num_v * num_v + c * num_v + c] = 1;
+            auto device_apply = [&](const std::vector<uint32_t>& in, std::vector<uint32_t>& out) {
+                check(pgw_kernel_vectors(op, in.data(), gens2.data(), 2, out.data(), prod.data()));
+            };
+            const uint64_t salts[3] = {0x404ULL, 0x505ULL, 0x606ULL};
+            std::vector<std::vector<uint32_t>> sx(3);
+            for (int idx = 0; idx < 3; ++idx) {
+                std::vector<uint32_t> in(cols * num_v), cpu(cols * num_v), gpu(cols * num_v);
+                check(pgw_deterministic_block(cols, num_v, p, salts[idx], in.data()));
+                device_apply(in, gpu);
+                if (inst.synthetic) {  // no host apply for the synthetic operator: symmetry only
+                    sx[idx] = std::move(gpu);
+                    continue;
+                }
+                check(pgw_cyclic_apply_cpu(&desc, dr.data(), dc.data(), rounds, pairs.data(), in.data(), num_v,
+                                           cpu.data()));
+                for (uint64_t q = 0; q < cpu.size(); ++q)
+                    if (cpu[q] != gpu[q])
+                        throw std::runtime_error(op_name + " square block " + std::to_string(idx) + " mismatch at (" +
+                                                 std::to_string(q / num_v) + "," + std::to_string(q % num_v) +
+                                                 "): CPU=" + std::to_string(cpu[q]) +
+                                                 ", GPU=" + std::to_string(gpu[q]));
+                std::cout << "Validation matched: " << op_name << " square block " << idx << " (" << cols << " x "
+                          << num_v << ")" << std::endl;
+                sx[idx] = std::move(gpu);
             }
-            std::cout << "Validation matched: <S u, w> = <u, S w> on 3 salted blocks" << std::endl;
+            std::vector<uint32_t> u0(cols * num_v), u1(cols * num_v);
+            check(pgw_deterministic_block(cols, num_v, p, salts[0], u0.data()));
+            check(pgw_deterministic_block(cols, num_v, p, salts[1], u1.data()));
+            if (dot_mod(sx[0], u1, p) != dot_mod(u0, sx[1], p))
+                throw std::runtime_error("GPU square operator is not symmetric on blocks 0 and 1");
+            std::cout << "Validation matched: <S u, w> = <u, S w> on blocks 0 and 1" << std::endl;
         }
 
         const uint64_t max_len = has(a, "--length") ? get_u64(a, "--length", 0)

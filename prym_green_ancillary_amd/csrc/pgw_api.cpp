@@ -193,6 +193,10 @@ hipStream_t pick_stream(pgw_op* op, void* stream) {
     return stream ? (hipStream_t)stream : op->ctx->stream;
 }
 
+}  // namespace
+
+namespace pgw {
+
 void validate_desc_common(const pgw_operator_desc* d) {
     require(d != nullptr, "null operator description");
     require(d->prime > 2 && d->prime < (1u << 31), "prime must satisfy 2 < p < 2^31");
@@ -200,6 +204,10 @@ void validate_desc_common(const pgw_operator_desc* d) {
         require(d->prime % q != 0, "modulus " + std::to_string(d->prime) + " is not prime");
     require(d->block >= 1 && d->block <= 16, "block width must be in 1..16");
 }
+
+}  // namespace pgw
+
+namespace {
 
 void synthetic_dims(const pgw_operator_desc* d, uint64_t& rows, uint64_t& cols, uint64_t& mid) {
     require(d->n >= 2 && d->n < 64, "synthetic operator needs 2 <= n < 64");
@@ -211,6 +219,10 @@ void synthetic_dims(const pgw_operator_desc* d, uint64_t& rows, uint64_t& cols, 
     rows = cols;
     mid = ut * d->m_tgt;
 }
+
+}  // namespace
+
+namespace pgw {
 
 bool is_deformation(uint32_t kind) {
     return kind == PGW_KIND_CYCLIC_AUGMENTED || kind == PGW_KIND_CYCLIC_REPLACEMENT;
@@ -233,7 +245,41 @@ uint64_t deformation_cols(const pgw_operator_desc* d, uint64_t base, uint32_t& d
     return base - 2 + d->dense_cols;
 }
 
-}  // namespace
+int64_t validate_rowmix_plan(uint64_t rows, uint32_t rounds, const pgw_rowmix_pair* pairs, uint32_t p) {
+    const uint64_t pc = rows / 2;
+    if (rounds == 0 || pc == 0) return -1;
+    require(pairs != nullptr, "row-mix plan is required when rounds > 0");
+    std::vector<uint8_t> seen(rows);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        std::fill(seen.begin(), seen.end(), 0);
+        for (uint64_t q = 0; q < pc; ++q) {
+            const pgw_rowmix_pair& pr = pairs[(uint64_t)r * pc + q];
+            require(pr.first < rows && pr.second < rows && pr.first != pr.second,
+                    "row-mix pair outside the target rows");
+            require(!seen[pr.first] && !seen[pr.second], "row-mix pairs of a round must be disjoint");
+            require(pr.a >= 1 && pr.a < p && pr.b >= 1 && pr.b < p,
+                    "row-mix coefficients must be nonzero residues");
+            seen[pr.first] = seen[pr.second] = 1;
+        }
+    }
+    int64_t unpaired = -1;
+    if (rows % 2 == 1)
+        for (uint64_t q = 0; q < rows; ++q)
+            if (!seen[q]) unpaired = (int64_t)q;  // left out of the last round
+    return unpaired;
+}
+
+void validate_preconditioners(const uint32_t* dr, uint64_t rows, const uint32_t* dc, uint64_t cols,
+                              uint32_t p) {
+    require(dr && dc, "preconditioners are required");
+    uint32_t lo, hi;
+    min_max_u32(dr, rows, lo, hi);
+    require(rows == 0 || (lo >= 1 && hi < p), "row preconditioner entries must be nonzero residues");
+    min_max_u32(dc, cols, lo, hi);
+    require(cols == 0 || (lo >= 1 && hi < p), "column preconditioner entries must be nonzero residues");
+}
+
+}  // namespace pgw
 
 // ---------------------------------------------------------------------------
 
@@ -443,11 +489,7 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             throw Error(PGW_ERR_INVALID, "unknown operator kind");
         }
         // preconditioners and the row-mixing plan
-        uint32_t lo, hi;
-        min_max_u32(h_row_precond, op->rows, lo, hi);
-        require(op->rows == 0 || (lo >= 1 && hi < op->p), "row preconditioner entries must be nonzero residues");
-        min_max_u32(h_col_precond, op->cols, lo, hi);
-        require(op->cols == 0 || (lo >= 1 && hi < op->p), "column preconditioner entries must be nonzero residues");
+        validate_preconditioners(h_row_precond, op->rows, h_col_precond, op->cols, op->p);
         op->d_dr = A.upload(h_row_precond, op->rows);
         op->d_dc = A.upload(h_col_precond, op->cols);
         std::vector<uint32_t> negdc(op->cols);
@@ -456,25 +498,8 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
         op->rounds = rowmix_rounds;
         op->pair_count = op->rows / 2;
         if (rowmix_rounds > 0 && op->pair_count > 0) {
-            require(h_pairs != nullptr, "row-mix plan is required when rounds > 0");
             const uint64_t total = (uint64_t)rowmix_rounds * op->pair_count;
-            std::vector<uint8_t> seen(op->rows);
-            for (uint32_t r = 0; r < rowmix_rounds; ++r) {
-                std::fill(seen.begin(), seen.end(), 0);
-                for (uint64_t q = 0; q < op->pair_count; ++q) {
-                    const pgw_rowmix_pair& pr = h_pairs[(uint64_t)r * op->pair_count + q];
-                    require(pr.first < op->rows && pr.second < op->rows && pr.first != pr.second,
-                            "row-mix pair outside the target rows");
-                    require(!seen[pr.first] && !seen[pr.second], "row-mix pairs of a round must be disjoint");
-                    require(pr.a >= 1 && pr.a < op->p && pr.b >= 1 && pr.b < op->p,
-                            "row-mix coefficients must be nonzero residues");
-                    seen[pr.first] = seen[pr.second] = 1;
-                }
-            }
-            op->unpaired = -1;
-            if (op->rows % 2 == 1)
-                for (uint64_t q = 0; q < op->rows; ++q)
-                    if (!seen[q]) op->unpaired = (int64_t)q;  // left out of the last round
+            op->unpaired = validate_rowmix_plan(op->rows, rowmix_rounds, h_pairs, op->p);
             op->d_pairs = A.upload(h_pairs, total);
         } else {
             op->rounds = 0;

@@ -213,6 +213,46 @@ struct CyclicLayout {
 };
 void build_cyclic_layout(CyclicLayout& L, const pgw_operator_desc& d);
 
+// Entries (column, coefficient) of output row `row` of a character-restricted
+// differential's forward gather (the index rules of
+// cyclic_phi_forward_scaled_kernel, cuda/cyclic_cuda_helpers.h:145-204): for
+// each incidence (i, s' + i) and each a in the character block of
+// w(beta) - w(i), the coefficient +-mu[i][beta][a] on input column
+// colbase(s' + i) + local_a.
+template <typename Emit>
+void phi_forward_row(const PhiHost& h, uint64_t row, uint32_t p, Emit&& emit) {
+    const uint32_t beta = h.row_beta[row];
+    const uint32_t bw = h.wA1[beta];
+    const uint64_t s = h.row_rank[row];
+    for (uint32_t pos = 0; pos < h.row_width; ++pos) {
+        const uint32_t code = h.row_inc_code[s * h.row_width + pos];
+        const uint32_t i = code & 0x7fffffffu;
+        const bool neg = code >> 31;
+        const uint32_t q = (bw + h.r - (h.wV[i] % h.r)) % h.r;
+        const uint32_t start = h.a0_off[q], end = h.a0_off[q + 1];
+        const uint64_t base = h.col_off[h.row_inc_rank[s * h.row_width + pos]];
+        const uint32_t* murow = h.mu.data() + ((uint64_t)i * h.nB + beta) * h.nA;
+        for (uint32_t idx = start; idx < end; ++idx) {
+            const uint32_t t = murow[h.a0_idx[idx]];
+            if (t == 0) continue;
+            emit(base + (idx - start), neg ? p - t : t);
+        }
+    }
+}
+
+// Descriptor and input checks shared by the device operator and the host
+// apply (pgw_api.cpp).
+void validate_desc_common(const pgw_operator_desc* d);
+bool is_deformation(uint32_t kind);
+// source columns of a deformation variant over a base of `base` columns;
+// drop0 < drop1 are the replacement's dropped base columns
+uint64_t deformation_cols(const pgw_operator_desc* d, uint64_t base, uint32_t& drop0, uint32_t& drop1);
+// disjointness, range and coefficients of a row-mixing plan; returns the row
+// left out of the rounds when rows is odd, else -1
+int64_t validate_rowmix_plan(uint64_t rows, uint32_t rounds, const pgw_rowmix_pair* pairs, uint32_t p);
+void validate_preconditioners(const uint32_t* dr, uint64_t rows, const uint32_t* dc, uint64_t cols,
+                              uint32_t p);
+
 // Explicit sparse maps (pgw_sparse.cpp): CSR with sorted columns and nonzero
 // canonical coefficients, and its sliced-ELL form for the device (slices of
 // kSellHeight rows; entry k of row r of slice s at ent[2 (slice[s] +
@@ -239,5 +279,9 @@ struct CyclicMaps {
 void build_cyclic_maps(CyclicMaps& out, const CyclicLayout& L, const pgw_operator_desc& d,
                        uint64_t cols, const uint32_t* dc, const uint32_t* dr, uint32_t rounds,
                        const pgw_rowmix_pair* pairs);
+// out = S in on a cols x w host block, matrix-free from the layout (pgw_cpu.cpp)
+void cyclic_apply_host(const CyclicLayout& L, const pgw_operator_desc& d, uint64_t cols, const uint32_t* dr,
+                       const uint32_t* dc, uint32_t rounds, const pgw_rowmix_pair* pairs, const uint32_t* in,
+                       uint32_t w, uint32_t* out);
 
 }  // namespace pgw

@@ -95,33 +95,6 @@ CsrHost flatten(RowLists& L, uint64_t cols, uint32_t p) {
     return M;
 }
 
-// Entries (column, coefficient) of output row `row` of a character-restricted
-// differential's forward gather (the index rules of
-// cyclic_phi_forward_scaled_kernel, cuda/cyclic_cuda_helpers.h:145-204): for
-// each incidence (i, s' + i) and each a in the character block of
-// w(beta) - w(i), the coefficient +-mu[i][beta][a] on input column
-// colbase(s' + i) + local_a.
-template <typename Emit>
-void phi_forward_row(const PhiHost& h, uint64_t row, uint32_t p, Emit&& emit) {
-    const uint32_t beta = h.row_beta[row];
-    const uint32_t bw = h.wA1[beta];
-    const uint64_t s = h.row_rank[row];
-    for (uint32_t pos = 0; pos < h.row_width; ++pos) {
-        const uint32_t code = h.row_inc_code[s * h.row_width + pos];
-        const uint32_t i = code & 0x7fffffffu;
-        const bool neg = code >> 31;
-        const uint32_t q = (bw + h.r - (h.wV[i] % h.r)) % h.r;
-        const uint32_t start = h.a0_off[q], end = h.a0_off[q + 1];
-        const uint64_t base = h.col_off[h.row_inc_rank[s * h.row_width + pos]];
-        const uint32_t* murow = h.mu.data() + ((uint64_t)i * h.nB + beta) * h.nA;
-        for (uint32_t idx = start; idx < end; ++idx) {
-            const uint32_t t = murow[h.a0_idx[idx]];
-            if (t == 0) continue;
-            emit(base + (idx - start), neg ? p - t : t);
-        }
-    }
-}
-
 }  // namespace
 
 CsrHost csr_transpose(const CsrHost& A) {

@@ -136,3 +136,19 @@ def test_recovery_cli_certificate(tmp_path, g12, g12_reference):
     (tmp_path / "bad_generators.txt").write_text("1 0 0 1\n")
     r = run([REC, G12, "-f", bad, "--sector", sector, "--rowmix-rounds", "2"])
     assert r.returncode == 1 and "certificate failed" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("instance,sector,b", [("g12", 0, 4), ("g12", 4, 3), ("g20", 0, 4)])
+def test_generator_cli_validate_rowmix_compares_device_and_host_exactly(tmp_path, instance, sector, b):
+    """--validate-rowmix (validate_eliminated_cpu_cuda, cuda/cuprym_cyclic.cu:
+    1970-2132): the device S X equals the exact host apply on the salted
+    blocks 0x404, 0x505, 0x606, entry by entry."""
+    print(f"validate-rowmix {instance} sector {sector} b={b}", flush=True)
+    r = run([GEN, ROOT / "tests" / "golden" / f"{instance}.json", "--sector", sector, "-v", b,
+             "--validate-rowmix", "--no-save", "--length", 9, "--out", tmp_path])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Validation matched: eliminated target rowmix transpose identity" in r.stdout
+    for idx in range(3):
+        assert f"Validation matched: eliminated square block {idx} (" in r.stdout, r.stdout[-2000:]
+    assert "Validation matched: <S u, w> = <u, S w>" in r.stdout

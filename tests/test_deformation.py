@@ -312,3 +312,21 @@ def test_gpu_deformation_sequence_rank(g12, g12_reference, gctx, variant, sector
     d, prod = op.kernel_vectors(v, r.generator().reshape(-1, b * b))
     exact = o.cols - 2 if variant == "augmented" else g12_reference["completion_rank"]
     assert bool(prod.any()) == (rank != exact)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,sector", [("augmented", 5), ("replacement", 0)])
+def test_generator_cli_deformation_validate_rowmix(tmp_path, g12, g12_reference, variant, sector):
+    # the device apply of a deformation variant equals the exact host apply
+    print(f"validate-rowmix {variant} sector {sector}", flush=True)
+    p = g12["modulus"]
+    dense, drops = _variant_setup(g12, g12_reference, variant, sector)
+    df, dcf = _write_inputs(tmp_path, p, dense, drops if variant == "replacement" else None)
+    args = [G12, "--sector", sector, "--out", tmp_path, "-v", 4, "--operator", variant, "--dense-file", df,
+            "--validate-rowmix", "--no-save", "--length", 9]
+    if dcf is not None:
+        args += ["--drop-columns", dcf]
+    r = _cli(args)
+    assert r.returncode == 0, r.stderr[-2000:]
+    for idx in range(3):
+        assert f"Validation matched: {variant} square block {idx} (" in r.stdout, r.stdout[-2000:]

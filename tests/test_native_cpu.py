@@ -23,8 +23,23 @@ def test_library_exports_every_declared_symbol():
     assert set(pg.EXPORTED) == declared
 
 
+def test_one_hip_runtime_per_process():
+    # libpgw shares the HIP and HSA runtimes torch brought in: a process
+    # holds one copy of each, whichever of the two was asked for first
+    import subprocess
+    import sys
+    code = ("import prym_green_ancillary_amd as pg; pg.lib(); import torch; "
+            "m = open('/proc/self/maps').read().splitlines(); "
+            "print(len({l.split()[-1] for l in m if 'libamdhip64' in l}), "
+            "len({l.split()[-1] for l in m if 'libhsa-runtime64' in l}))")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                         cwd=str(HEADER.parents[1]))
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert out.stdout.split() == ["1", "1"]
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
-    monkeypatch.setenv("PGW_LIBRARY", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(pg, "LIB_PATH", tmp_path / "absent.so")
     monkeypatch.setattr(pg, "_lib", None)
     with pytest.raises(pg.NativeLibraryError):
         pg.lib()
