@@ -288,7 +288,7 @@ def bench_serve(args):
 
     def rank_summary(sel):
         st = [r[3].status for r, f in zip(mixed, is_full) if f == sel]
-        ranks = [int(r[3].rank) for r, f in zip(mixed, is_full) if f == sel]
+        ranks = [int(r[3].rank_lower_bound) for r, f in zip(mixed, is_full) if f == sel]
         return {"jobs": len(st),
                 "status_counts": {str(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
                 "ranks": ranks if sel else None}

@@ -152,7 +152,6 @@ void forward_into_target(pgw_op* op, const uint32_t* x, hipStream_t st) {
 
 // out = Dc F^T (op->d_target), synthetic operator
 void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
-    const ModP& m = op->m;
     if (op->method == PGW_METHOD_MFMA) {
         timed_limb_split(op, op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, st);
         run_sweep(op, 2, nullptr, op->d_lp_src, op->d_mid, nullptr, st);

@@ -19,6 +19,7 @@
 #include <mutex>
 #include <utility>
 
+#include "pgw_fold.h"
 #include "pgw_kernels.h"
 
 namespace pgw {
@@ -332,9 +333,7 @@ void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uin
 template <bool M31>
 __device__ __forceinline__ uint32_t red63(uint64_t v, const ModP& m) {
     if constexpr (M31) {
-        uint64_t r = (v & 0x7fffffffULL) + (v >> 31);  // < 2^32 + 2^31
-        r = (r & 0x7fffffffULL) + (r >> 31);           // < 2^31 + 3
-        return (uint32_t)(r >= 0x7fffffffULL ? r - 0x7fffffffULL : r);
+        return red63_m31(v);
     } else {
         return mod_u64(v, m);
     }

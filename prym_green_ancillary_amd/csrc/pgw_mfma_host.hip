@@ -125,14 +125,8 @@ __device__ __forceinline__ void limb_split_one(const uint32_t* __restrict__ in,
             const uint64_t row = v * k_in + k;
             x = in[row * B + c];
             if (scale) {
-                if (m.p == 0x7fffffffu) {  // x * scale < 2^62: two Mersenne folds leave at most p + 1
-                    uint64_t t = (uint64_t)x * scale[row];
-                    t = (t & 0x7fffffffULL) + (t >> 31);
-                    t = (t & 0x7fffffffULL) + (t >> 31);
-                    x = (uint32_t)(t >= m.p ? t - m.p : t);
-                } else {
-                    x = mulmod(x, scale[row], m);
-                }
+                // x * scale < 2^62
+                x = m.p == kM31 ? red63_m31((uint64_t)x * scale[row]) : mulmod(x, scale[row], m);
             }
         }
         const uint32_t xn = x ? m.p - x : 0u;

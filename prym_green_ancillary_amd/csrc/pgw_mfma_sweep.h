@@ -34,6 +34,7 @@
 #include <utility>
 #include <vector>
 
+#include "pgw_fold.h"
 #include "pgw_kernels.h"
 
 namespace pgw {
@@ -48,21 +49,6 @@ __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int
 template <int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-struct DigitSpec {
-    int64_t maxpos;
-    uint64_t bias;
-};
-
-__host__ __device__ inline DigitSpec digit_spec(uint32_t nl) {
-    return nl == 2 ? DigitSpec{32639, 0x8080ULL} : DigitSpec{2139062143LL, 0x80808080ULL};
-}
-
-// bytes d_j + 128 of the representative; XOR 0x80 per byte gives the int8 digit
-__host__ __device__ inline uint64_t digits_biased(uint32_t x, uint32_t p, const DigitSpec& ds) {
-    int64_t y = (int64_t)x <= ds.maxpos ? (int64_t)x : (int64_t)x - (int64_t)p;
-    return (uint64_t)(y + (int64_t)ds.bias);
 }
 
 constexpr int kLdsStride = 64;  // bytes per staged row (one 64-wide K chunk)
@@ -124,14 +110,7 @@ __device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t
 }
 
 // Epilogue: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile; fold
-// the digit shifts to one residue, scale, store.  |acc_s| <= 2^31.
-//  * p = 2^31 - 1: 256^s == 2^(8s mod 31), so sum_s acc_s 2^(8s mod 31) is
-//    the value mod p and is exact in 64 bits: |sum| <= 2^31 (1 + 2 + 2^8 +
-//    2^9 + 2^16 + 2^17 + 2^24) < 2^56 - 2^25 = p 2^25; offset by p 2^25 it
-//    lies in (0, 2^57) and folds twice (2^31 == 1) to at most p + 1.
-//  * two digits (p <= 65536): sum_s acc_s 256^s is exact, |sum| < 2^48 <=
-//    p 2^47; offset by p 2^47 it is one 64-bit Barrett reduction.
-//  * otherwise: 64-bit Barrett per shift and weights w_s = 256^s mod p.
+// the digit shifts to one residue (fold_digit_shifts), scale, store.
 // The lane's scale factors (1 without a scale vector), all loaded before the
 // epilogue's first store: a load after a store to `out` would wait for it
 // (the pointers may alias).  Loading them before the K loop instead costs 16
@@ -165,31 +144,7 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
     const int wr = wave / WC, wc = wave % WC;
     const int cc = lane % 16;
     const int my_slot = cc / B, my_c = cc % B;
-    const uint32_t p = a.m.p;
-    const bool mersenne = NL == 4 && p == 0x7fffffffu;
-    // (f & p) + (f >> 31) == f (mod 2^31 - 1)
-    auto fold31 = [](uint64_t f) { return (f & 0x7fffffffULL) + (f >> 31); };
-    auto fold = [&](const int32_t (&x)[NS]) -> uint32_t {
-        if (NL == 4 && mersenne) {
-            uint64_t t = (uint64_t)p << 25;
-            static_for<NS>([&](auto s) { t += (uint64_t)(int64_t)x[s] << ((8 * decltype(s)::value) % 31); });
-            const uint32_t v = (uint32_t)fold31(fold31(t));
-            return v >= p ? v - p : v;
-        } else if (NL == 2) {
-            uint64_t t = (uint64_t)p << 47;
-            static_for<NS>([&](auto s) { t += (uint64_t)(int64_t)x[s] << (8 * decltype(s)::value); });
-            return mod_u64(t, a.m);
-        } else {
-            const uint64_t off = (uint64_t)p << 32;  // multiple of p above 2^31
-            uint64_t s_lo = 0, s_hi = 0;
-            static_for<NS>([&](auto s) {
-                const uint32_t r = mod_u64((uint64_t)((int64_t)x[s] + (int64_t)off), a.m);
-                const uint64_t prod = (uint64_t)r * a.sw.w[s];  // < 2^62
-                if (s < 4) s_lo += prod; else s_hi += prod;     // 4 terms < 2^64
-            });
-            return addmod(mod_u64(s_lo, a.m), mod_u64(s_hi, a.m), p);
-        }
-    };
+    const bool mersenne = NL == 4 && a.m.p == kM31;
     static_for<RT>([&](auto rt) {
         static_for<CT>([&](auto ct) {
             const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
@@ -198,16 +153,11 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
                 if (u >= a.u_out || row >= a.r_out) return;
                 int32_t x[NS];
                 static_for<NS>([&](auto s) { x[s] = acc[rt][ct][s][decltype(v)::value]; });
-                uint32_t val = fold(x);
+                uint32_t val = fold_digit_shifts<NL>(x, a.m, a.sw.w);
                 const uint64_t orow = u * a.r_out + row;
                 if (a.scale) {
                     const uint32_t f = sc[rt][ct][v];
-                    if (mersenne) {  // product < 2^62: two folds leave at most p + 1
-                        const uint32_t q = (uint32_t)fold31(fold31((uint64_t)val * f));
-                        val = q >= p ? q - p : q;
-                    } else {
-                        val = mulmod(val, f, a.m);
-                    }
+                    val = mersenne ? red63_m31((uint64_t)val * f) : mulmod(val, f, a.m);  // product < 2^62
                 }
                 a.out[orow * B + my_c] = val;
             });
