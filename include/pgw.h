@@ -290,7 +290,9 @@ typedef struct pgw_batch pgw_batch;
 
 enum pgw_batch_flags {
     PGW_BATCH_GRAPH = 1,    /* replay a captured graph of the round (else launch every replay) */
-    PGW_BATCH_GROUPED = 2   /* one launch per stage for each group of compatible jobs */
+    PGW_BATCH_GROUPED = 2,  /* one launch per stage for each group of compatible jobs */
+    PGW_BATCH_AUTOTUNE = 4  /* time tile configurations 7-9 once per operator shape (n, k, M, M',
+                               b, limbs, fold) and give every job of that shape the fastest */
 };
 
 int pgw_batch_create(pgw_ctx* ctx, uint32_t flags, pgw_batch** out);
@@ -318,7 +320,7 @@ int pgw_batch_destroy(pgw_batch* b);
 
 /* ------------------------------------------------------------------------
  * Serving: sector jobs admitted by their device-memory footprint and run
- * as one continuously refilled, cost-balanced batch.
+ * as one continuously refilled batch.
  *
  * Replaces the reference's resource-pool scheduling of sector processes
  * (scripts/reproduce.py:78-111, 246-279) for sector jobs.  Admission is

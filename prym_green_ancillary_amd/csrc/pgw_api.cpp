@@ -282,6 +282,60 @@ void validate_preconditioners(const uint32_t* dr, uint64_t rows, const uint32_t*
 
 // ---------------------------------------------------------------------------
 
+namespace pgw {
+
+void autotune_sweeps(pgw_op* op, uint32_t reps, int cfg_lo, int cfg_hi, uint32_t* h_choice, double* h_ms) {
+    PGW_HIP(hipSetDevice(op->ctx->device));
+    hipStream_t st = op->ctx->stream;
+    if (op->method != PGW_METHOD_MFMA) {
+        for (int w = 0; w < 4; ++w) {
+            if (h_choice) h_choice[w] = 0;
+            if (h_ms) h_ms[w] = 0.0;
+        }
+        return;
+    }
+    if (reps == 0) reps = 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PGW_HIP(hipEventCreate(&e0));
+    try {
+        PGW_HIP(hipEventCreate(&e1));
+        // the sweeps' schedules do not depend on the data: time them on the
+        // operator's own digit planes and scratch blocks
+        for (int w = 0; w < 4; ++w) {
+            const DenseSweep s = sweep_desc(op, w);
+            const int8_t* lp = s.fwd ? op->d_lp_src : op->d_lp_mid;
+            uint32_t* out = s.fwd ? op->d_mid : op->d_target;
+            double best = 1e30;
+            int best_cfg = op->mfma_cfg[w];
+            for (int c = cfg_lo; c <= cfg_hi; ++c) {
+                launch_dense_mfma(s, op->mt[w], lp, out, nullptr, op->b, op->nl, op->m, st, c);  // warm
+                PGW_HIP(hipEventRecord(e0, st));
+                for (uint32_t r = 0; r < reps; ++r)
+                    launch_dense_mfma(s, op->mt[w], lp, out, nullptr, op->b, op->nl, op->m, st, c);
+                PGW_HIP(hipEventRecord(e1, st));
+                PGW_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                PGW_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / reps < best) {
+                    best = ms / reps;
+                    best_cfg = c;
+                }
+            }
+            op->mfma_cfg[w] = best_cfg;
+            if (h_choice) h_choice[w] = (uint32_t)best_cfg;
+            if (h_ms) h_ms[w] = best;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+}  // namespace pgw
+
 extern "C" {
 
 int pgw_ctx_create(int device, pgw_ctx** out) {
@@ -723,48 +777,7 @@ int pgw_kernel_vectors(pgw_op* op, const uint32_t* h_v, const uint32_t* h_gens, 
 int pgw_operator_autotune(pgw_op* op, uint32_t reps, uint32_t* h_choice, double* h_ms) {
     return guarded([&] {
         require(op != nullptr, "null operator");
-        PGW_HIP(hipSetDevice(op->ctx->device));
-        hipStream_t st = op->ctx->stream;
-        const int count = mfma_config_count();
-        if (op->method != PGW_METHOD_MFMA) {
-            for (int w = 0; w < 4; ++w) {
-                if (h_choice) h_choice[w] = 0;
-                if (h_ms) h_ms[w] = 0.0;
-            }
-            return;
-        }
-        if (reps == 0) reps = 1;
-        hipEvent_t e0, e1;
-        PGW_HIP(hipEventCreate(&e0));
-        PGW_HIP(hipEventCreate(&e1));
-        // the sweeps' schedules do not depend on the data: time them on the
-        // operator's own digit planes and scratch blocks
-        for (int w = 0; w < 4; ++w) {
-            const DenseSweep s = sweep_desc(op, w);
-            const int8_t* lp = s.fwd ? op->d_lp_src : op->d_lp_mid;
-            uint32_t* out = s.fwd ? op->d_mid : op->d_target;
-            double best = 1e30;
-            int best_cfg = op->mfma_cfg[w];
-            for (int c = 0; c < count; ++c) {
-                launch_dense_mfma(s, op->mt[w], lp, out, nullptr, op->b, op->nl, op->m, st, c);  // warm
-                PGW_HIP(hipEventRecord(e0, st));
-                for (uint32_t r = 0; r < reps; ++r)
-                    launch_dense_mfma(s, op->mt[w], lp, out, nullptr, op->b, op->nl, op->m, st, c);
-                PGW_HIP(hipEventRecord(e1, st));
-                PGW_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                PGW_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms / reps < best) {
-                    best = ms / reps;
-                    best_cfg = c;
-                }
-            }
-            op->mfma_cfg[w] = best_cfg;
-            if (h_choice) h_choice[w] = (uint32_t)best_cfg;
-            if (h_ms) h_ms[w] = best;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        autotune_sweeps(op, reps, 0, mfma_config_count() - 1, h_choice, h_ms);
     });
 }
 

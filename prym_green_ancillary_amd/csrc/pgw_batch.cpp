@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -265,6 +266,10 @@ struct pgw_batch {
     std::mutex mtx, step_mtx;
     std::map<uint64_t, BatchJob> jobs;
     uint64_t next_id = 1;
+    // PGW_BATCH_AUTOTUNE: tile configurations (7-9) per operator shape, timed
+    // on the first operator of that shape and reused for the others
+    std::mutex tune_mtx;
+    std::map<std::string, std::array<int, 4>> tuned;
     // the plan of the current set of unfinished jobs (under step_mtx)
     std::vector<uint64_t> plan_key;
     std::vector<Group> groups;
@@ -372,11 +377,40 @@ struct pgw_batch {
 
 extern "C" {
 
+namespace {
+
+// The operator's shape as far as the sweeps' schedules go: subsets, panel
+// heights, block width, limb count and fold path.
+std::string shape_key(const pgw_op* op) {
+    return std::to_string(op->n) + "/" + std::to_string(op->k) + "/" + std::to_string(op->M) + "/" +
+           std::to_string(op->Mp) + "/" + std::to_string(op->b) + "/" + std::to_string(op->nl) + "/" +
+           (op->p == 0x7fffffffu ? "m31" : "bar");
+}
+
+// Tile configurations of a new job's operator: the cached choice for its
+// shape, or the fastest of the groupable configurations 7-9 timed on this
+// operator (its buffers are not yet in use by any job).
+void tune_for_shape(pgw_batch* b, pgw_op* op) {
+    const std::string key = shape_key(op);
+    std::lock_guard<std::mutex> tl(b->tune_mtx);
+    auto it = b->tuned.find(key);
+    if (it == b->tuned.end()) {
+        autotune_sweeps(op, 2, 7, 9, nullptr, nullptr);
+        std::array<int, 4> c{};
+        for (int w = 0; w < 4; ++w) c[w] = op->mfma_cfg[w];
+        it = b->tuned.emplace(key, c).first;
+    }
+    for (int w = 0; w < 4; ++w) op->mfma_cfg[w] = it->second[w];
+}
+
+}  // namespace
+
 int pgw_batch_create(pgw_ctx* ctx, uint32_t flags, pgw_batch** out) {
     pgw_batch* b = nullptr;
     int rc = guarded([&] {
         require(ctx && out, "null context or output pointer");
-        require((flags & ~(uint32_t)(PGW_BATCH_GRAPH | PGW_BATCH_GROUPED)) == 0, "unknown batch flags");
+        require((flags & ~(uint32_t)(PGW_BATCH_GRAPH | PGW_BATCH_GROUPED | PGW_BATCH_AUTOTUNE)) == 0,
+                "unknown batch flags");
         PGW_HIP(hipSetDevice(ctx->device));
         b = new pgw_batch;
         b->ctx = ctx;
@@ -417,6 +451,8 @@ int pgw_batch_add(pgw_batch* b, pgw_op* op, const uint32_t* h_v, uint64_t max_le
         j.rounds_total = (max_len - 1 + 3) / 4;
         j.L = 1 + 4 * j.rounds_total;
         PGW_HIP(hipSetDevice(b->ctx->device));
+        if ((b->flags & PGW_BATCH_AUTOTUNE) && op->kind == PGW_KIND_SYNTHETIC && op->method == PGW_METHOD_MFMA)
+            tune_for_shape(b, op);
         {
             std::lock_guard<std::mutex> cl(b->ctx->capture_mtx);
             PGW_HIP(hipMalloc(&j.d_B, N * bb * sizeof(uint32_t)));

@@ -136,4 +136,10 @@ namespace pgw {
 // d_out = S d_in on stream st (the block-apply chain, enqueue only).
 void apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st);
 
+// Times tile configurations cfg_lo..cfg_hi of each of the four MFMA sweeps on
+// the operator's own buffers (reps launches each, HIP events on the
+// context's stream) and keeps the fastest per sweep in op->mfma_cfg; plain
+// operators report configuration 0 and 0 ms.
+void autotune_sweeps(pgw_op* op, uint32_t reps, int cfg_lo, int cfg_hi, uint32_t* h_choice, double* h_ms);
+
 }  // namespace pgw
