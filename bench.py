@@ -120,6 +120,39 @@ def cpu_baseline(spec, T, E, dr, dc, pairs, v, b, budget_s=20.0):
     }
 
 
+def cpu_baseline_cyclic(inst, sector, b, budget_s=20.0):
+    """The oracle's apply chain of one cyclic eliminated sector (the paper
+    recipe), whole block-applies repeated for about budget_s, at one host
+    thread and at all (at most 16)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_ctypes as oc
+
+    L = oc.lib()
+    nthreads = min(16, os.cpu_count() or 1)
+    p = inst["modulus"]
+    o = oc.cyclic_op(inst, sector)
+    dr, dc, v = oc.reference_random_state(1, sector, p, o.rows, o.cols, b)
+    _, pairs, _ = oc.rowmix_plan(10001, "eliminated", sector, o.rows, p, 2)
+    o.set_preconditioners(dr, dc, 2, pairs)
+
+    def rate(threads, budget):
+        L.orc_set_threads(threads)
+        x, n, t0 = v, 0, time.perf_counter()
+        while True:
+            x = o.apply_square(x, b)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget:
+                return n / el, n
+
+    r1, n1 = rate(1, budget_s * 0.4)
+    rm, nm = rate(nthreads, budget_s * 0.6)
+    return {"value": rm, "unit": "block-applies/s", "cores": nthreads, "kind": "port",
+            "single_thread_value": r1,
+            "sample": f"oracle apply_square of sector {sector} ({o.rows} x {o.cols}, b={b}): "
+                      f"{nm} whole block-applies at {nthreads} OpenMP threads, {n1} at 1 thread"}
+
+
 def traffic_from_profiles(workload):
     """HBM bytes per launch of the dominant kernel from the committed counter
     summary profiles/traffic_<workload>.txt (`key value` lines: FETCH_SIZE and
@@ -405,6 +438,7 @@ def bench_paper(args):
               file=sys.stderr, flush=True)
     med = float(np.median(loops))
     pub = case["published_base_gpu_time_median_seconds"]
+    cpu = None if args.no_cpu_baseline else cpu_baseline_cyclic(inst, sectors[0], b, args.cpu_budget)
     line = {
         "metric": f"median base-sector sequence-loop time, genus {args.genus} (the reference's "
                   "published metric for its paper instances)",
@@ -420,7 +454,7 @@ def bench_paper(args):
         "median_block_applies_per_s": round(float(np.median([x["block_applies_per_s"] for x in per_sector])), 1),
         "per_sector": per_sector,
         "device": dev_name,
-        "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
@@ -451,7 +485,7 @@ def main():
     ap.add_argument("--max-active", type=int, default=16, help="serve: concurrent jobs")
     ap.add_argument("--hbm-fraction", type=float, default=0.5, help="serve: HBM budget share")
     ap.add_argument("--no-rank", action="store_true", help="serve: skip the host rank stage")
-    ap.add_argument("--rank-threads", type=int, default=2, help="serve: rank-stage threads per job")
+    ap.add_argument("--rank-threads", type=int, default=8, help="serve: rank-stage threads per job")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); the sector path "

@@ -553,7 +553,14 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
         if (rowmix_rounds > 0 && op->pair_count > 0) {
             const uint64_t total = (uint64_t)rowmix_rounds * op->pair_count;
             op->unpaired = validate_rowmix_plan(op->rows, rowmix_rounds, h_pairs, op->p);
-            op->d_pairs = A.upload(h_pairs, total);
+            // the pairs of a round are disjoint, so they may run in any order:
+            // sorted by first row, half of every round's row accesses stream
+            std::vector<pgw_rowmix_pair> sorted(h_pairs, h_pairs + total);
+            for (uint32_t r = 0; r < rowmix_rounds; ++r)
+                std::sort(sorted.begin() + (uint64_t)r * op->pair_count,
+                          sorted.begin() + (uint64_t)(r + 1) * op->pair_count,
+                          [](const pgw_rowmix_pair& x, const pgw_rowmix_pair& y) { return x.first < y.first; });
+            op->d_pairs = A.upload(sorted);
         } else {
             op->rounds = 0;
         }

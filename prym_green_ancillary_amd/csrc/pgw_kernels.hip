@@ -343,41 +343,78 @@ __device__ __forceinline__ uint32_t red63(uint64_t v, const ModP& m) {
 // (cuda/cyclic_cuda_helpers.h:109-143).  `mid` fuses forward, Dr and transpose
 // of the same round: the 2x2 block P_r^T diag(dr_f, dr_s) P_r on each pair.
 // Every sum below is u*x + v*y with residues below 2^31, so below 2^63.
+//
+// Each lane moves CPL = 4 columns of both rows with 16-byte loads and stores
+// when B is a multiple of 4 (one column otherwise), so the B / CPL lanes of a
+// pair cover its two rows with one coalesced access each and a wave keeps
+// 64 CPL / B pairs in flight.  (One lane per pair with four 16-byte loads per
+// row ran slower: every load instruction then touches 64 scattered lines.)
+template <int B>
+struct MixShape {
+    static constexpr int CPL = B % 4 == 0 ? 4 : 1;  // columns per lane
+    static constexpr int LPP = B / CPL;             // lanes per pair
+    static constexpr int PPB = 256 / LPP;           // pairs per 256-thread block
+};
+
 template <int B, int MODE, bool M31>  // MODE 0 forward, 1 transpose, 2 fused middle
-__device__ __forceinline__ void rowmix_pair_one(const pgw_rowmix_pair& pr, uint32_t* __restrict__ block,
-                                                const uint32_t* __restrict__ dr, const ModP& m, int c) {
+__device__ __forceinline__ void rowmix_pair_cols(const pgw_rowmix_pair& pr, uint32_t* __restrict__ block,
+                                                 const uint32_t* __restrict__ dr, const ModP& m, int lane) {
+    constexpr int CPL = MixShape<B>::CPL;
     const uint32_t p = m.p;
     const uint32_t opab = addmod(1u, red63<M31>((uint64_t)pr.a * pr.b, m), p);
-    uint32_t* fx = block + (uint64_t)pr.first * B + c;
-    uint32_t* sy = block + (uint64_t)pr.second * B + c;
-    uint64_t x = *fx, y = *sy;
-    uint32_t nx, ny;
-    if (MODE == 1) {
-        nx = red63<M31>(x + (uint64_t)pr.b * y, m);
-        ny = red63<M31>((uint64_t)pr.a * x + (uint64_t)opab * y, m);
+    uint32_t* fr = block + (uint64_t)pr.first * B + lane * CPL;
+    uint32_t* sr = block + (uint64_t)pr.second * B + lane * CPL;
+    uint32_t x[CPL], y[CPL];
+    if constexpr (CPL == 4) {
+        const uint4 a = *reinterpret_cast<const uint4*>(fr), c = *reinterpret_cast<const uint4*>(sr);
+        x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+        y[0] = c.x; y[1] = c.y; y[2] = c.z; y[3] = c.w;
     } else {
-        nx = red63<M31>(x + (uint64_t)pr.a * y, m);
-        ny = red63<M31>((uint64_t)pr.b * x + (uint64_t)opab * y, m);
-        if (MODE == 2) {
-            const uint64_t sx = red63<M31>((uint64_t)nx * dr[pr.first], m);
-            const uint64_t sy2 = red63<M31>((uint64_t)ny * dr[pr.second], m);
-            nx = red63<M31>(sx + (uint64_t)pr.b * sy2, m);
-            ny = red63<M31>((uint64_t)pr.a * sx + (uint64_t)opab * sy2, m);
-        }
+        x[0] = *fr;
+        y[0] = *sr;
     }
-    *fx = nx;
-    *sy = ny;
+    uint32_t dr_f = 0, dr_s = 0;
+    if constexpr (MODE == 2) {
+        dr_f = dr[pr.first];
+        dr_s = dr[pr.second];
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        const uint64_t xc = x[c], yc = y[c];
+        uint32_t nx, ny;
+        if constexpr (MODE == 1) {
+            nx = red63<M31>(xc + (uint64_t)pr.b * yc, m);
+            ny = red63<M31>((uint64_t)pr.a * xc + (uint64_t)opab * yc, m);
+        } else {
+            nx = red63<M31>(xc + (uint64_t)pr.a * yc, m);
+            ny = red63<M31>((uint64_t)pr.b * xc + (uint64_t)opab * yc, m);
+            if constexpr (MODE == 2) {
+                const uint64_t sx = red63<M31>((uint64_t)nx * dr_f, m);
+                const uint64_t sy = red63<M31>((uint64_t)ny * dr_s, m);
+                nx = red63<M31>(sx + (uint64_t)pr.b * sy, m);
+                ny = red63<M31>((uint64_t)pr.a * sx + (uint64_t)opab * sy, m);
+            }
+        }
+        x[c] = nx;
+        y[c] = ny;
+    }
+    if constexpr (CPL == 4) {
+        *reinterpret_cast<uint4*>(fr) = make_uint4(x[0], x[1], x[2], x[3]);
+        *reinterpret_cast<uint4*>(sr) = make_uint4(y[0], y[1], y[2], y[3]);
+    } else {
+        *fr = x[0];
+        *sr = y[0];
+    }
 }
 
 template <int B, int MODE, bool M31>
 __global__ __launch_bounds__(256) void k_rowmix(const pgw_rowmix_pair* __restrict__ pairs,
                                                 uint64_t count, uint32_t* __restrict__ block,
                                                 const uint32_t* __restrict__ dr, ModP m) {
-    constexpr int RPB = 256 / B;
-    const uint64_t q = (uint64_t)blockIdx.x * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB || q >= count) return;
-    rowmix_pair_one<B, MODE, M31>(pairs[q], block, dr, m, c);
+    constexpr int LPP = MixShape<B>::LPP, PPB = MixShape<B>::PPB;
+    const uint64_t q = (uint64_t)blockIdx.x * PPB + threadIdx.x / LPP;
+    if ((int)(threadIdx.x / LPP) >= PPB || q >= count) return;
+    rowmix_pair_cols<B, MODE, M31>(pairs[q], block, dr, m, (int)(threadIdx.x % LPP));
 }
 
 // many jobs' row-mixing rounds in one launch; slot `count` of a job (one past
@@ -386,17 +423,17 @@ template <int B, int MODE, bool M31>
 __global__ __launch_bounds__(256) void k_rowmix_group(const MixJob* __restrict__ jobs,
                                                       const uint32_t* __restrict__ first,
                                                       uint32_t njobs, uint32_t round) {
-    constexpr int RPB = 256 / B;
+    constexpr int CPL = MixShape<B>::CPL, LPP = MixShape<B>::LPP, PPB = MixShape<B>::PPB;
     const uint32_t j = group_job(first, njobs);
     const MixJob& J = jobs[j];
-    const uint64_t q = (uint64_t)(blockIdx.x - first[j]) * RPB + threadIdx.x / B;
-    const int c = threadIdx.x % B;
-    if (threadIdx.x / B >= RPB) return;
+    const int lane = (int)(threadIdx.x % LPP);
+    if ((int)(threadIdx.x / LPP) >= PPB) return;
+    const uint64_t q = (uint64_t)(blockIdx.x - first[j]) * PPB + threadIdx.x / LPP;
     if (q < J.count) {
-        rowmix_pair_one<B, MODE, M31>(J.pairs[(uint64_t)round * J.count + q], J.block, J.dr, J.m, c);
+        rowmix_pair_cols<B, MODE, M31>(J.pairs[(uint64_t)round * J.count + q], J.block, J.dr, J.m, lane);
     } else if (MODE == 2 && q == J.count && J.unpaired >= 0) {
-        uint32_t* x = J.block + (uint64_t)J.unpaired * B + c;
-        *x = mulmod(*x, J.dr[J.unpaired], J.m);
+        uint32_t* x = J.block + (uint64_t)J.unpaired * B + lane * CPL;
+        for (int c = 0; c < CPL; ++c) x[c] = mulmod(x[c], J.dr[J.unpaired], J.m);
     }
 }
 
@@ -411,7 +448,7 @@ template <int B, bool M31>
 static void target_precond_bm(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pc,
                               uint32_t rounds, const uint32_t* dr, int64_t unpaired_row,
                               const ModP& m, hipStream_t st) {
-    dim3 grid(grid_for(pc, 256 / B)), blk(256);
+    dim3 grid(grid_for(pc, MixShape<B>::PPB)), blk(256);
     for (uint32_t r = 0; r + 1 < rounds; ++r)
         hipLaunchKernelGGL((k_rowmix<B, 0, M31>), grid, blk, 0, st, pairs + (uint64_t)r * pc, pc,
                            block, dr, m);
@@ -435,8 +472,9 @@ static void target_precond_b(uint32_t* block, const pgw_rowmix_pair* pairs, uint
 }
 
 uint64_t mix_group_blocks(const MixJob& j, uint32_t b) {
-    const uint64_t rpb = 256 / b;
-    return (j.count + 1 + rpb - 1) / rpb;  // one slot past the pairs: the unpaired row
+    // pairs per block (MixShape<b>::PPB); one slot past the pairs: the unpaired row
+    const uint64_t ppb = 256 / (b % 4 == 0 ? b / 4 : b);
+    return (j.count + 1 + ppb - 1) / ppb;
 }
 
 template <int B>

@@ -164,6 +164,20 @@ def test_mfma_tile_configs_vs_oracle(torch_cuda, ctx, monkeypatch, cfg, p, b, n,
     assert (gpu_apply(torch_cuda, op, v) == o.apply_square(v, b)).all()
 
 
+@pytest.mark.parametrize("cfg", [7, 8, 9])
+@pytest.mark.parametrize("p,b,n,k,M,Mp", [(P31, 16, 6, 3, 140, 150), (65521, 8, 6, 2, 136, 290),
+                                          (P31, 4, 5, 2, 270, 300)])
+def test_mfma_short_last_row_slice_vs_oracle(torch_cuda, ctx, monkeypatch, cfg, p, b, n, k, M, Mp):
+    # padded rows that leave a short last row slice, whose workgroups hold
+    # one or two row tiles (144 = 128 + 16, 160 = 128 + 32, 304 = 256 + 48,
+    # 272 = 256 + 16 rows)
+    spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=23)
+    o = oc.synthetic_op(n, k, M, Mp, p, T, E, dr, dc, 2, pairs.reshape(-1))
+    monkeypatch.setenv("PGW_MFMA_CFG", str(cfg))
+    op = pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=pg.METHOD_MFMA, tables=(T, E))
+    assert (gpu_apply(torch_cuda, op, v) == o.apply_square(v, b)).all()
+
+
 def test_autotune_keeps_results(torch_cuda, ctx):
     n, k, M, Mp, p, b = 9, 4, 40, 44, P31, 16
     spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=2)
