@@ -466,6 +466,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--loop-rounds", type=int, default=8,
+                    help="sector workloads: rounds of the timed sequence loop (4 terms each)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", choices=["sector", "large", "batch", "serve", "paper"], default="large",
                     help="large: the N=53.8M sector (default); sector: the mid-size sector; "
@@ -551,6 +553,15 @@ def main():
         lat.append((time.perf_counter() - t1) * 1000.0)
         a, bb = bb, a
 
+    # the sequence loop itself (pgw_sequence_run: V upload, V^T V, captured
+    # rounds of two applies and four Gram projections, results back), the
+    # quantity the reference times; block-applies/s with the projections in
+    loop_len = 1 + 4 * max(1, args.loop_rounds)
+    t0 = time.perf_counter()
+    op.sequence(v, loop_len)
+    loop_s = time.perf_counter() - t0
+    loop_rate = (loop_len - 1) / 2 / loop_s
+
     # plain u64 path on the same workload
     plain_ms, _, _ = plain.bench_apply(a.data_ptr(), bb.data_ptr(), max(1, args.plain_steps))
     plain_rate = 1000.0 / (plain_ms / max(1, args.plain_steps))
@@ -607,6 +618,10 @@ def main():
                                f"p={'2^31-1' if p == P31 else p} b={b} rowmix 2 rounds",
                    "n": n, "k": k, "M": M,
                    "M_prime": Mp, "N": N, "prime": p, "block": b, "parallelism": "single GPU"},
+        "sequence_loop": {"terms": loop_len, "s": round(loop_s, 3),
+                          "block_applies_per_s": round(loop_rate, 3),
+                          "note": "pgw_sequence_run wall time: V upload and final copy-back "
+                                  "included, Gram projections included"},
         "plain_block_applies_per_s": round(plain_rate, 3),
         "mfma_vs_plain": round(value / plain_rate, 3),
         "bit_exact_mfma_vs_plain": True,
