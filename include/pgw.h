@@ -221,7 +221,11 @@ typedef void (*pgw_progress_fn)(uint64_t done, uint64_t total, void* user);
 /* Whole sequence loop (cuprym_cyclic.cu:2316-2450): seq[0] = V^T V, then
  * ceil((max_len-1)/4) rounds of {D=S B; B^T D; D^T D; B=S D; B^T D; B^T B}.
  * h_seq receives L x b(b+1)/2 values with L = 1+4*ceil((max_len-1)/4)
- * (row t = packed V^T S^t V), h_curv the final N x b block.  *out_len = L. */
+ * (row t = packed V^T S^t V), h_curv the final N x b block.  *out_len = L.
+ * Every 64 rounds the terms finished so far are copied to h_seq before
+ * progress(done, L, user) is called, so rows [0, done) of h_seq hold their
+ * final values inside the callback (a streaming WDM writer reads them while
+ * the loop goes on; the host memory of later rows is written only after). */
 int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t* h_seq,
                      uint32_t* h_curv, uint64_t* out_len, pgw_progress_fn progress,
                      void* user);
@@ -443,6 +447,29 @@ int pgw_wdm_write(const char* path, uint64_t rows, uint64_t cols, uint32_t prime
                   const uint32_t* h_dr, const uint32_t* h_dc, const uint32_t* h_v,
                   const uint32_t* h_curv, uint32_t block, const uint32_t* h_seq, uint64_t length,
                   uint32_t threads);
+
+/* Streaming form of pgw_wdm_write for a sequence that is still being
+ * computed: the same file, byte for byte, with the text written while the
+ * loop runs instead of after it (the reference formats and compresses on one
+ * thread after the loop, cuda/cuprym_cyclic.cu:965-1007 called at :2461).
+ * _open starts a writer thread that formats and compresses the header, Dr,
+ * Dc and the V lines at once (h_dr, h_dc, h_v must stay valid until _finish
+ * or _destroy).  _terms(h_seq, done): rows [0, done) of h_seq (length x
+ * block(block+1)/2, the same array at every call) hold their final values;
+ * the writer turns them into the entry lines' text in memory (up to
+ * max_text_bytes of it; longer sequences are formatted at _finish).  The
+ * progress callback of pgw_sequence_run is called exactly when this holds.
+ * _finish(h_curv) writes curV and the entry lines, closes the zstd frame and
+ * renames path.tmp to path.  _destroy of an unfinished stream stops the
+ * writer and removes path.tmp.  Errors: PGW_ERR_IO. */
+typedef struct pgw_wdm_stream pgw_wdm_stream;
+int pgw_wdm_stream_open(const char* path, uint64_t rows, uint64_t cols, uint32_t prime,
+                        const uint32_t* h_dr, const uint32_t* h_dc, const uint32_t* h_v,
+                        uint32_t block, uint64_t length, uint32_t threads, uint64_t max_text_bytes,
+                        pgw_wdm_stream** out);
+int pgw_wdm_stream_terms(pgw_wdm_stream* s, const uint32_t* h_seq, uint64_t done);
+int pgw_wdm_stream_finish(pgw_wdm_stream* s, const uint32_t* h_curv);
+int pgw_wdm_stream_destroy(pgw_wdm_stream* s);
 
 
 

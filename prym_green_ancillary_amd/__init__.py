@@ -106,6 +106,7 @@ EXPORTED = [
     "pgw_rank_get_info", "pgw_rank_delta", "pgw_rank_generator", "pgw_rank_destroy",
     "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_bench_stages",
     "pgw_gram_upper_noalloc", "pgw_cyclic_apply_cpu", "pgw_deterministic_block",
+    "pgw_wdm_stream_open", "pgw_wdm_stream_terms", "pgw_wdm_stream_finish", "pgw_wdm_stream_destroy",
 ]
 
 
@@ -196,6 +197,11 @@ def lib():
         "pgw_server_rank": (I, [vp, u64, ctypes.POINTER(RankInfo), vp, u64, ctypes.POINTER(u64),
                                 ctypes.POINTER(ctypes.c_double)]),
         "pgw_wdm_write": (I, [ctypes.c_char_p, u64, u64, u32, vp, vp, vp, vp, u32, vp, u64, u32]),
+        "pgw_wdm_stream_open": (I, [ctypes.c_char_p, u64, u64, u32, vp, vp, vp, u32, u64, u32, u64,
+                                    ctypes.POINTER(vp)]),
+        "pgw_wdm_stream_terms": (I, [vp, vp, u64]),
+        "pgw_wdm_stream_finish": (I, [vp, vp]),
+        "pgw_wdm_stream_destroy": (I, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -611,6 +617,53 @@ def write_wdm(path, rows: int, cols: int, p: int, dr, dc, v, curv, b: int, seq, 
         raise ValueError("WDM array shapes do not match rows, cols and b")
     _check(lib().pgw_wdm_write(str(path).encode(), rows, cols, p, _ptr(dr), _ptr(dc), _ptr(v),
                                _ptr(curv), b, _ptr(seq.reshape(-1)), L, threads))
+
+
+class WdmStream:
+    """The same WDM file as write_wdm, written while the sequence is still
+    being computed (pgw_wdm_stream_*): the prefix (header, Dr, Dc, V) at once,
+    sequence rows as terms(seq, done) hands them over, curV and the entry
+    lines at finish(curv).  `seq` must be the (L, b(b+1)/2) array whose rows
+    [0, done) are final at each terms() call."""
+
+    def __init__(self, path, rows: int, cols: int, p: int, dr, dc, v, b: int, length: int,
+                 threads: int = 0, max_text_bytes: int = 1 << 31):
+        self._keep = (_u32(dr).reshape(-1), _u32(dc).reshape(-1), _u32(v).reshape(-1))
+        dr, dc, v = self._keep
+        if dr.size != rows or dc.size != cols or v.size != cols * b:
+            raise ValueError("WDM array shapes do not match rows, cols and b")
+        self.length, self.ut, self.cols, self.b = length, b * (b + 1) // 2, cols, b
+        self.h = ctypes.c_void_p()
+        self._seq = None
+        _check(lib().pgw_wdm_stream_open(str(path).encode(), rows, cols, p, _ptr(dr), _ptr(dc), _ptr(v),
+                                         b, length, threads, max_text_bytes, ctypes.byref(self.h)))
+
+    def terms(self, seq, done: int):
+        if self._seq is None:
+            if seq.dtype != np.uint32 or not seq.flags.c_contiguous or seq.size != self.length * self.ut:
+                raise ValueError("seq must be a contiguous uint32 (L, b(b+1)/2) array")
+            self._seq = seq  # the writer reads it until finish
+        elif seq is not self._seq:
+            raise ValueError("terms() must be given the same array every time")
+        _check(lib().pgw_wdm_stream_terms(self.h, _ptr(seq), done))
+
+    def finish(self, curv):
+        curv = _u32(curv).reshape(-1)
+        if curv.size != self.cols * self.b or self._seq is None:
+            raise ValueError("finish needs the sequence and a (cols, b) curV block")
+        _check(lib().pgw_wdm_stream_finish(self.h, _ptr(curv)))
+        self.close()
+
+    def close(self):
+        if self.h:
+            lib().pgw_wdm_stream_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def job_footprint(desc: OperatorDesc, max_len: int, rounds: int = 2) -> int:

@@ -17,6 +17,7 @@
 #include <filesystem>
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <vector>
 
 #include "cli_common.h"
@@ -52,9 +53,18 @@ void host_rowmix(std::vector<uint32_t>& x, const std::vector<pgw_rowmix_pair>& p
     }
 }
 
+// the loop's progress: a status line, and the finished terms handed to the
+// streaming WDM writer (rows [0, done) of the sequence are final here)
+struct LoopState {
+    std::chrono::steady_clock::time_point start;
+    pgwio::WdmStream* stream = nullptr;
+    const uint32_t* seq = nullptr;
+};
+
 void progress_cb(uint64_t done, uint64_t total, void* user) {
-    auto* start = static_cast<std::chrono::steady_clock::time_point*>(user);
-    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - *start).count();
+    auto* st = static_cast<LoopState*>(user);
+    if (st->stream) st->stream->terms(st->seq, done);
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - st->start).count();
     double rate = secs > 0 ? done / secs : 0.0;
     std::cout << "\rProgress: " << done << "/" << total << " | Elapsed: " << (long long)secs
               << "s | Throughput: " << std::fixed << std::setprecision(2) << rate << "/s"
@@ -255,9 +265,18 @@ int main(int argc, char** argv) {
         std::cout << "Sequence target: " << max_len << " entries (" << L << " after rounding to rounds)"
                   << std::endl;
         std::vector<uint32_t> seq(L * ut), curv(cols * num_v);
-        auto t0 = std::chrono::steady_clock::now();
+        // the WDM text is written while the loop runs: header, Dr, Dc and V
+        // at once, the sequence terms as they come back (every 64 rounds)
+        std::unique_ptr<pgwio::WdmStream> stream;
+        if (!no_save) {
+            std::cout << "Streaming WDM state to " << wdm << " during the sequence loop" << std::endl;
+            stream.reset(new pgwio::WdmStream(wdm, rows, cols, inst.modulus, dr.data(), dc.data(), v.data(),
+                                              num_v, L));
+        }
+        LoopState loop{std::chrono::steady_clock::now(), stream.get(), seq.data()};
+        const auto t0 = loop.start;
         uint64_t got = 0;
-        check(pgw_sequence_run(op, v.data(), max_len, seq.data(), curv.data(), &got, progress_cb, &t0));
+        check(pgw_sequence_run(op, v.data(), max_len, seq.data(), curv.data(), &got, progress_cb, &loop));
         auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count();
         std::cout << "\nSector " << sector << " sequence runtime: " << ms << " ms" << std::endl;
         if (ms > 0)
@@ -266,10 +285,16 @@ int main(int argc, char** argv) {
         if (no_save) {
             std::cout << "Skipping final WDM save due to --no-save." << std::endl;
         } else {
-            std::cout << "Saving WDM state to " << wdm << "..." << std::endl;
-            pgwio::save_wdm_sym(wdm, rows, cols, inst.modulus, dr.data(), dc.data(), v.data(),
-                                curv.data(), num_v, seq.data(), got);
-            std::cout << "Saved " << got << " sequence elements to " << wdm << std::endl;
+            if (got != L) throw std::runtime_error("sequence length differs from the streamed WDM header");
+            std::cout << "Finishing WDM state " << wdm << "..." << std::endl;
+            auto tw = std::chrono::steady_clock::now();
+            stream->terms(seq.data(), got);
+            stream->finish(curv.data());
+            stream.reset();
+            std::cout << "Saved " << got << " sequence elements to " << wdm << " ("
+                      << std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - tw)
+                             .count()
+                      << " ms after the loop)" << std::endl;
         }
         check(pgw_operator_destroy(op));
         check(pgw_ctx_destroy(ctx));

@@ -719,14 +719,24 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
             PGW_HIP(hipStreamEndCapture(st, &graph));
             PGW_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
+        // rows [0, copied) of h_seq hold their final values
+        uint64_t copied = 0;
         for (uint64_t r = 0; r < rounds; ++r) {
             PGW_HIP(hipGraphLaunch(exec, st));
             if ((r + 1) % 64 == 0 || r + 1 == rounds) {
                 PGW_HIP(hipStreamSynchronize(st));  // also bounds the launch queue
-                if (progress) progress(1 + 4 * (r + 1), L, user);
+                if (progress) {
+                    const uint64_t done = 1 + 4 * (r + 1);
+                    PGW_HIP(hipMemcpy(h_seq + copied * ut, d_seq + copied * ut, (done - copied) * ut * sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost));
+                    copied = done;
+                    progress(done, L, user);
+                }
             }
         }
-        PGW_HIP(hipMemcpyAsync(h_seq, d_seq, L * ut * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        if (copied < L)
+            PGW_HIP(hipMemcpyAsync(h_seq + copied * ut, d_seq + copied * ut, (L - copied) * ut * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, st));
         PGW_HIP(hipMemcpyAsync(h_curv, d_B, N * b * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         PGW_HIP(hipStreamSynchronize(st));
         if (out_len) *out_len = L;

@@ -4,11 +4,13 @@
 
 #include <algorithm>
 #include <charconv>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -263,6 +265,20 @@ static void put_line(TextSink& s, const uint32_t* v, uint64_t count, uint64_t st
     s.put_char('\n');
 }
 
+// "m n p L b", Dr, Dc and the b lines of V: everything known before the loop
+static void put_prefix(TextSink& s, uint64_t rows, uint64_t cols, uint32_t p, const uint32_t* dr,
+                       const uint32_t* dc, const uint32_t* v, uint32_t b, uint64_t L, unsigned threads) {
+    s.put_u64(rows); s.put_char(' ');
+    s.put_u64(cols); s.put_char(' ');
+    s.put_u64(p); s.put_char(' ');
+    s.put_u64(L); s.put_char(' ');
+    s.put_u64(b); s.put_char('\n');
+    // preconditioners are written as stored (cuda/wdmfiles.h:195-199)
+    put_line(s, dr, rows, 1, 0, threads);
+    put_line(s, dc, cols, 1, 0, threads);
+    for (uint32_t c = 0; c < b; ++c) put_line(s, v + c, cols, b, p, threads);
+}
+
 void save_wdm_sym(const std::string& path, uint64_t rows, uint64_t cols, uint32_t p,
                   const uint32_t* dr, const uint32_t* dc, const uint32_t* v, const uint32_t* curv,
                   uint32_t b, const uint32_t* seq, uint64_t L, unsigned threads) {
@@ -271,21 +287,159 @@ void save_wdm_sym(const std::string& path, uint64_t rows, uint64_t cols, uint32_
     const std::string tmp = path + ".tmp";
     {
         TextSink s(tmp, path_requests_compression(tmp), threads);
-        s.put_u64(rows); s.put_char(' ');
-        s.put_u64(cols); s.put_char(' ');
-        s.put_u64(p); s.put_char(' ');
-        s.put_u64(L); s.put_char(' ');
-        s.put_u64(b); s.put_char('\n');
-        // preconditioners are written as stored (cuda/wdmfiles.h:195-199)
-        put_line(s, dr, rows, 1, 0, threads);
-        put_line(s, dc, cols, 1, 0, threads);
-        for (uint32_t c = 0; c < b; ++c) put_line(s, v + c, cols, b, p, threads);
+        put_prefix(s, rows, cols, p, dr, dc, v, b, L, threads);
         for (uint32_t c = 0; c < b; ++c) put_line(s, curv + c, cols, b, p, threads);
         const uint64_t ut = (uint64_t)b * (b + 1) / 2;
         for (uint64_t e = 0; e < ut; ++e) put_line(s, seq + e, L, ut, p, threads);
         s.finish();
     }
     std::filesystem::rename(tmp, path);
+}
+
+struct WdmStream::Impl {
+    std::string path, tmp;
+    uint64_t rows = 0, cols = 0, L = 0, ut = 0;
+    uint32_t p = 0, b = 0;
+    unsigned threads = 1;
+    const uint32_t *dr = nullptr, *dc = nullptr, *v = nullptr;
+    bool buffer_lines = false;
+    std::vector<std::string> lines;  // entry e's values of rows [0, formatted), space separated
+
+    std::mutex mu;
+    std::condition_variable cv;
+    const uint32_t* seq = nullptr;  // guarded by mu
+    const uint32_t* curv = nullptr;
+    uint64_t ready = 0;             // rows of seq with final values
+    bool finishing = false, stop = false;
+    std::exception_ptr err;
+    std::thread worker;
+
+    // rows [t0, t1) of every entry appended to its line
+    void format_rows(const uint32_t* s, uint64_t t0, uint64_t t1) {
+        char tmp_c[16];
+        for (uint64_t e = 0; e < ut; ++e) {
+            std::string& out = lines[e];
+            for (uint64_t t = t0; t < t1; ++t) {
+                if (t) out.push_back(' ');
+                auto r = std::to_chars(tmp_c, tmp_c + sizeof(tmp_c), s[t * ut + e] % p);
+                out.append(tmp_c, (size_t)(r.ptr - tmp_c));
+            }
+        }
+    }
+
+    void run() {
+        try {
+            TextSink s(tmp, path_requests_compression(tmp), threads);
+            put_prefix(s, rows, cols, p, dr, dc, v, b, L, threads);
+            uint64_t formatted = 0;
+            const uint32_t* sq = nullptr;
+            const uint32_t* cv_block = nullptr;
+            for (;;) {
+                uint64_t upto;
+                bool fin;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return stop || finishing || (buffer_lines && ready > formatted); });
+                    if (stop) return;
+                    upto = ready;
+                    fin = finishing;
+                    sq = seq;
+                    cv_block = curv;
+                }
+                if (buffer_lines && upto > formatted) {
+                    format_rows(sq, formatted, upto);
+                    formatted = upto;
+                }
+                if (fin) break;
+            }
+            for (uint32_t c = 0; c < b; ++c) put_line(s, cv_block + c, cols, b, p, threads);
+            for (uint64_t e = 0; e < ut; ++e) {
+                if (buffer_lines) {
+                    s.put(lines[e].data(), lines[e].size());
+                    s.put_char('\n');
+                    std::string().swap(lines[e]);
+                } else {
+                    put_line(s, sq + e, L, ut, p, threads);
+                }
+            }
+            s.finish();
+            std::filesystem::rename(tmp, path);
+        } catch (...) {
+            err = std::current_exception();
+        }
+    }
+};
+
+WdmStream::WdmStream(const std::string& path, uint64_t rows, uint64_t cols, uint32_t p, const uint32_t* dr,
+                     const uint32_t* dc, const uint32_t* v, uint32_t b, uint64_t L, unsigned threads,
+                     uint64_t max_text_bytes)
+    : impl_(new Impl) {
+    if (L == 0) throw std::runtime_error("Cannot save WDM file with empty sequence list");
+    Impl& m = *impl_;
+    m.path = path;
+    m.tmp = path + ".tmp";
+    m.rows = rows;
+    m.cols = cols;
+    m.p = p;
+    m.dr = dr;
+    m.dc = dc;
+    m.v = v;
+    m.b = b;
+    m.L = L;
+    m.ut = (uint64_t)b * (b + 1) / 2;
+    m.threads = threads ? threads : writer_threads();
+    // at most 11 characters per value (10 digits and a separator)
+    m.buffer_lines = L <= max_text_bytes / 11 / m.ut;
+    if (m.buffer_lines) {
+        m.lines.resize(m.ut);
+        for (auto& s : m.lines) s.reserve(L * 11);
+    }
+    m.worker = std::thread([this] { impl_->run(); });
+}
+
+WdmStream::~WdmStream() {
+    Impl& m = *impl_;
+    if (m.worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(m.mu);
+            m.stop = true;
+        }
+        m.cv.notify_all();
+        m.worker.join();
+        std::error_code ec;
+        std::filesystem::remove(m.tmp, ec);
+    }
+}
+
+void WdmStream::terms(const uint32_t* seq, uint64_t done) {
+    Impl& m = *impl_;
+    if (!seq) throw std::runtime_error("WDM stream: null sequence");
+    {
+        std::lock_guard<std::mutex> lk(m.mu);
+        if (m.finishing) throw std::runtime_error("WDM stream: terms after finish");
+        m.seq = seq;
+        m.ready = std::max(m.ready, std::min(done, m.L));
+    }
+    m.cv.notify_all();
+}
+
+void WdmStream::finish(const uint32_t* curv) {
+    Impl& m = *impl_;
+    if (!m.worker.joinable()) throw std::runtime_error("WDM stream: already finished");
+    {
+        std::lock_guard<std::mutex> lk(m.mu);
+        if (!curv || !m.seq) throw std::runtime_error("WDM stream: finish needs the sequence and curV");
+        m.curv = curv;
+        m.ready = m.L;
+        m.finishing = true;
+    }
+    m.cv.notify_all();
+    m.worker.join();
+    if (m.err) {
+        std::error_code ec;
+        std::filesystem::remove(m.tmp, ec);
+        std::rethrow_exception(m.err);
+    }
 }
 
 namespace {

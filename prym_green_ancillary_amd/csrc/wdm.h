@@ -50,6 +50,33 @@ void save_wdm_sym(const std::string& path, uint64_t rows, uint64_t cols, uint32_
                   const uint32_t* dr, const uint32_t* dc, const uint32_t* v, const uint32_t* curv,
                   uint32_t b, const uint32_t* seq, uint64_t L, unsigned threads = 0);
 
+// The same file as save_wdm_sym, written while the sequence is computed.
+// The constructor starts a writer thread that formats and compresses the
+// header, Dr, Dc and the V lines at once (they are known before the loop).
+// terms(seq, done) says that rows [0, done) of `seq` (L x b(b+1)/2, the
+// caller's array, which must stay alive until finish) hold their final
+// values; the writer formats them into the b(b+1)/2 entry lines in memory
+// while later rows are still being computed (up to `max_text_bytes` of such
+// text; a longer sequence is formatted at the end instead).  finish(curv)
+// writes the curV lines and the entry lines, closes the frame and renames
+// the file; its bytes equal save_wdm_sym's for the same inputs.  Destroying
+// an unfinished stream stops the writer and removes the .tmp file.
+class WdmStream {
+public:
+    WdmStream(const std::string& path, uint64_t rows, uint64_t cols, uint32_t p, const uint32_t* dr,
+              const uint32_t* dc, const uint32_t* v, uint32_t b, uint64_t L, unsigned threads = 0,
+              uint64_t max_text_bytes = 1ull << 31);
+    ~WdmStream();
+    WdmStream(const WdmStream&) = delete;
+    WdmStream& operator=(const WdmStream&) = delete;
+    void terms(const uint32_t* seq, uint64_t done);
+    void finish(const uint32_t* curv);
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
 struct WdmInitial {
     uint64_t rows = 0, cols = 0, length = 0;
     uint32_t prime = 0, block = 0;

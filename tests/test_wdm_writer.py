@@ -85,3 +85,48 @@ def test_writer_errors(tmp_path):
         pg.write_wdm(tmp_path / "missing" / "x.wdm", 3, 4, 109, dr, dc, v, curv, 1, seq)
     with pytest.raises(ValueError):
         pg.write_wdm(tmp_path / "x.wdm", 3, 5, 109, dr, dc, v, curv, 1, seq)
+
+
+@pytest.mark.parametrize("rows,cols,L,b,p,cap", [
+    (11, 13, 41, 4, 661, 1 << 31),           # entry lines formatted as the terms arrive
+    (5, 6, 9, 1, 109, 1 << 31),
+    (11, 13, 41, 4, 661, 0),                 # over the text cap: formatted at finish
+    (2_000, 3_000, 20_001, 16, 2147483647, 1 << 31),
+])
+def test_streamed_writer_text_equals_writer(tmp_path, rows, cols, L, b, p, cap):
+    # the sequence arrives in uneven slices while the prefix is being written,
+    # as pgw_sequence_run's progress callback hands it over; the bytes must be
+    # those of the one-shot writer (itself pinned to the oracle above)
+    dr, dc, v, curv, seq = sample(rows, cols, L, b, p, L)
+    want_path = tmp_path / "whole.wdm"
+    pg.write_wdm(want_path, rows, cols, p, dr, dc, v, curv, b, seq)
+    want = want_path.read_text()
+    rng = np.random.default_rng(L)
+    for name in ("streamed.wdm", "streamed.wdm.zst"):
+        out = tmp_path / name
+        live = np.zeros_like(seq)  # rows become final only when handed over
+        s = pg.WdmStream(out, rows, cols, p, dr, dc, v, b, L, threads=3, max_text_bytes=cap)
+        done = 0
+        while done < L:
+            step = int(rng.integers(1, max(2, L // 5)))
+            live[done:done + step] = seq[done:done + step]
+            done = min(L, done + step)
+            s.terms(live, done)
+        assert not out.exists()
+        s.finish(curv)
+        text = one_frame_text(out) if name.endswith(".zst") else out.read_text()
+        assert text == want, name
+        assert not (tmp_path / (name + ".tmp")).exists()
+
+
+def test_streamed_writer_abandoned_leaves_no_file(tmp_path):
+    dr, dc, v, curv, seq = sample(3, 4, 9, 2, 109, 1)
+    out = tmp_path / "x.wdm.zst"
+    s = pg.WdmStream(out, 3, 4, 109, dr, dc, v, 2, 9)
+    s.terms(seq, 5)
+    s.close()  # the loop failed: no partial output survives
+    assert not out.exists() and not (tmp_path / "x.wdm.zst.tmp").exists()
+    with pytest.raises(pg.PgwError, match="failed to open"):
+        bad = pg.WdmStream(tmp_path / "missing" / "x.wdm", 3, 4, 109, dr, dc, v, 2, 9)
+        bad.terms(seq, 9)
+        bad.finish(curv)
