@@ -37,7 +37,7 @@ struct MfmaSweepArgs {
     uint32_t r_out, r_pad, k_in, k_pad;
     uint32_t row_slices;
     uint64_t col_groups;
-    uint32_t xcd_affinity;  // 1: grid is 8 * ceil(col_groups / (8 / row_slices)) blocks
+    uint64_t xcd_items;     // work items per XCD: the grid is 8 * xcd_items blocks
     uint32_t* out;
     const uint32_t* scale;
     ModP m;

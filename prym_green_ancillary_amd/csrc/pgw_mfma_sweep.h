@@ -63,22 +63,35 @@ __device__ __forceinline__ int lds_off(int row, int k) {
 constexpr int kMaxN = 32;
 
 // Workgroup -> (column group, row slice), XCD-aware: hardware block ids are
-// dealt round-robin to the 8 XCDs.  When the row slices divide 8, XCD x works
-// only on row slice x % R, so its L2 holds just that slice of the tables; its
-// column groups are spread over the 8/R XCDs sharing the slice.  The grid is
-// padded to a multiple of 8; returns false for the surplus blocks, which
-// leave before any barrier.
+// dealt round-robin to the 8 XCDs, so XCD x runs blocks x, x + 8, x + 16, ...
+// The col_groups x row_slices work items, in row-slice-major order, are cut
+// into 8 contiguous ranges of xcd_items, one per XCD: an XCD meets one or two
+// row slices, so its L2 holds just that part of the tables, whatever the
+// number of slices, and it walks neighbouring output panels in dispatch
+// order.  The grid is 8 * xcd_items blocks; returns false for the surplus
+// blocks, which leave before any barrier.
 __device__ __forceinline__ bool mfma_block_tile(const MfmaSweepArgs& a, uint32_t block, uint64_t& cg,
                                                 uint32_t& rs) {
-    if (a.xcd_affinity) {
-        const uint32_t xcd = block % 8, within = block / 8, G = 8 / a.row_slices;
-        rs = xcd % a.row_slices;
-        cg = (uint64_t)within * G + xcd / a.row_slices;
-        return cg < a.col_groups;
-    }
-    cg = block / a.row_slices;
-    rs = block % a.row_slices;
-    return cg < a.col_groups;
+    const uint64_t item = (uint64_t)(block % 8) * a.xcd_items + block / 8;
+    if (item >= a.col_groups * a.row_slices) return false;
+    rs = (uint32_t)(item / a.col_groups);
+    cg = item % a.col_groups;
+    return true;
+}
+
+// The output's row tiles (16 rows each, r_pad / 16 of them) are spread
+// evenly over the row slices: slice rs starts at row tile
+// rs * base + min(rs, extra) and holds base tiles, one more for the first
+// `extra` slices (base = tiles / slices, extra = tiles % slices).  A
+// workgroup costs about the same staging whatever its row count, so a short
+// last slice (c4's transpose: 400 rows = 128 + 128 + 128 + 16) would pay
+// nearly a full slice for a few rows.  base + 1 never exceeds a workgroup's
+// row tiles, since slices = ceil(tiles / (row tiles per workgroup)).
+__device__ __forceinline__ void mfma_slice_rows(const MfmaSweepArgs& a, uint32_t rs, uint32_t& row0,
+                                                uint32_t& tiles) {
+    const uint32_t all = a.r_pad / 16, base = all / a.row_slices, extra = all % a.row_slices;
+    row0 = 16 * (rs * base + (rs < extra ? rs : extra));
+    tiles = base + (rs < extra ? 1u : 0u);
 }
 
 // Incidence codes of the workgroup's output panels, read from the sweep's
@@ -117,7 +130,7 @@ __device__ __forceinline__ void mfma_incidences(const MfmaSweepArgs& a, uint64_t
 // VGPRs there, which makes the register-A kernels spill.
 template <int B, int RT, int CT, int WR, int WC>
 __device__ __forceinline__ void mfma_scales(const MfmaSweepArgs& a, uint64_t cg, uint32_t wg_row0,
-                                            uint32_t (&sc)[RT][CT][4]) {
+                                            uint32_t wg_tiles, uint32_t (&sc)[RT][CT][4]) {
     constexpr int SPT = 16 / B;
     constexpr int SLOTS = WC * CT * SPT;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
@@ -128,7 +141,8 @@ __device__ __forceinline__ void mfma_scales(const MfmaSweepArgs& a, uint64_t cg,
             const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
             static_for<4>([&](auto v) {
                 const uint32_t row = wg_row0 + (wr * RT + rt) * 16 + (lane / 16) * 4 + v;
-                sc[rt][ct][v] = a.scale && u < a.u_out && row < a.r_out ? a.scale[u * a.r_out + row] : 1u;
+                const bool mine = (uint32_t)(wr * RT + rt) < wg_tiles;
+                sc[rt][ct][v] = a.scale && mine && u < a.u_out && row < a.r_out ? a.scale[u * a.r_out + row] : 1u;
             });
         });
     });
@@ -136,7 +150,8 @@ __device__ __forceinline__ void mfma_scales(const MfmaSweepArgs& a, uint64_t cg,
 
 template <int B, int NL, int RT, int CT, int WR, int WC>
 __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, uint32_t wg_row0,
-                                           v4i (&acc)[RT][CT][2 * NL - 1], const uint32_t (&sc)[RT][CT][4]) {
+                                           uint32_t wg_tiles, v4i (&acc)[RT][CT][2 * NL - 1],
+                                           const uint32_t (&sc)[RT][CT][4]) {
     constexpr int NS = 2 * NL - 1;
     constexpr int SPT = 16 / B;
     constexpr int SLOTS = WC * CT * SPT;
@@ -150,7 +165,8 @@ __device__ __forceinline__ void mfma_store(const MfmaSweepArgs& a, uint64_t cg, 
             const uint64_t u = cg * SLOTS + (wc * CT + ct) * SPT + my_slot;
             static_for<4>([&](auto v) {
                 const uint32_t row = wg_row0 + (wr * RT + rt) * 16 + (lane / 16) * 4 + v;
-                if (u >= a.u_out || row >= a.r_out) return;
+                // rows of this slice only: the tiles after wg_tiles belong to the next one
+                if ((uint32_t)(wr * RT + rt) >= wg_tiles || u >= a.u_out || row >= a.r_out) return;
                 int32_t x[NS];
                 static_for<NS>([&](auto s) { x[s] = acc[rt][ct][s][decltype(v)::value]; });
                 uint32_t val = fold_digit_shifts<NL>(x, a.m, a.sw.w);
@@ -201,7 +217,8 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
     if (!mfma_block_tile(a, block, cg, rs)) return;
     const int tid = threadIdx.x, wave = tid / 64;
     const int wr = wave / WC, wc = wave % WC;
-    const uint32_t wg_row0 = rs * ROWS;
+    uint32_t wg_row0, wg_tiles;
+    mfma_slice_rows(a, rs, wg_row0, wg_tiles);
     mfma_incidences<SLOTS, SPT, NT>(a, cg, s_code, s_masks);
 
     v4i acc[RT][CT][NS];
@@ -241,7 +258,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
         const int item = tid + q * NT;
         const int rest = item / 4;  // rest = j*ROWS + r
         const int r = rest % ROWS, j = rest / ROWS;
-        a_ok[q] = item < A_ITEMS && wg_row0 + r < a.r_pad;
+        a_ok[q] = item < A_ITEMS && (uint32_t)r < wg_tiles * 16;
         a_loff[q] = (uint64_t)j * limb_block_a;
         a_row[q] = wg_row0 + r;
         a_soff[q] = lds_off(rest, piece * 16);
@@ -304,7 +321,7 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
     // row tiles of this wave that hold table rows (wave-uniform)
     bool row_on[RT];
     static_for<RT>([&](auto rt) {
-        row_on[rt] = __builtin_amdgcn_readfirstlane((int)(wg_row0 + (wr * RT + rt) * 16 < a.r_pad)) != 0;
+        row_on[rt] = __builtin_amdgcn_readfirstlane((int)((uint32_t)(wr * RT + rt) < wg_tiles)) != 0;
     });
 
     // chunk stream: full chunks of every incident table, then tail pairs
@@ -399,8 +416,8 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma(MfmaSweepArgs a)
         k0 = nk;
     }
     uint32_t sc[RT][CT][4];
-    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, sc);
-    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc, sc);
+    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, wg_tiles, sc);
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, wg_tiles, acc, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,7 +509,8 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
     if (!mfma_block_tile(a, block, cg, rs)) return;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int wr = wave / WC, wc = wave % WC;
-    const uint32_t wg_row0 = rs * ROWS;
+    uint32_t wg_row0, wg_tiles;
+    mfma_slice_rows(a, rs, wg_row0, wg_tiles);
     mfma_incidences<SLOTS, SPT, NT>(a, cg, s_code, s_masks);
 
     v4i acc[RT][CT][NS];
@@ -546,7 +564,7 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
             if (all_a || (!all_b && blk < A_BLOCKS)) {  // table rows: staged row = j * ROWS + r
                 const int r = row_s % ROWS, j = row_s / ROWS;
                 const uint32_t row = wg_row0 + r;
-                const uint32_t off = (t < n && row < a.r_pad)
+                const uint32_t off = (t < n && (uint32_t)r < wg_tiles * 16)
                                          ? (uint32_t)t * table_stride + k0 * a.r_pad + j * limb_block_a +
                                                (row << wshift) + poff
                                          : zero_off;
@@ -595,7 +613,7 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
 
     bool row_on[RT];
     static_for<RT>([&](auto rt) {
-        row_on[rt] = __builtin_amdgcn_readfirstlane((int)(wg_row0 + (wr * RT + rt) * 16 < a.r_pad)) != 0;
+        row_on[rt] = __builtin_amdgcn_readfirstlane((int)((uint32_t)(wr * RT + rt) < wg_tiles)) != 0;
     });
 
     const int cc = lane % 16, kq = (lane / 16) * 16;
@@ -680,8 +698,8 @@ void k_dense_mfma_dma(MfmaSweepArgs a) {
         }
     }
     uint32_t sc[RT][CT][4];
-    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, sc);
-    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc, sc);
+    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, wg_tiles, sc);
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, wg_tiles, acc, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -706,7 +724,6 @@ __device__ __forceinline__ void dense_mfma_areg_body(const MfmaSweepArgs& a, uin
     constexpr int SPT = 16 / B;
     constexpr int TILES_C = WC * CT;
     constexpr int SLOTS = TILES_C * SPT;
-    constexpr int ROWS = WR * RT * 16;
     constexpr int B_BYTES = TILES_C * NL * 16 * kLdsStride;
     constexpr int B_BLOCKS = B_BYTES / 1024;
     constexpr int PER = (B_BLOCKS + NW - 1) / NW;  // staging loads per wave and chunk (at most)
@@ -726,7 +743,8 @@ __device__ __forceinline__ void dense_mfma_areg_body(const MfmaSweepArgs& a, uin
     if (!mfma_block_tile(a, block, cg, rs)) return;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int wr = wave / WC, wc = wave % WC;
-    const uint32_t wg_row0 = rs * ROWS;
+    uint32_t wg_row0, wg_tiles;
+    mfma_slice_rows(a, rs, wg_row0, wg_tiles);
     mfma_incidences<SLOTS, SPT, NT>(a, cg, s_code, s_masks);
     // s_bptr[g * (kMaxN + 1) + t]: base of the digit plane (sign and limb 0)
     // that panel g reads at table t, or the zero table when g is not incident
@@ -769,7 +787,7 @@ __device__ __forceinline__ void dense_mfma_areg_body(const MfmaSweepArgs& a, uin
     };
     bool row_on[RT];
     static_for<RT>([&](auto rt) {
-        row_on[rt] = __builtin_amdgcn_readfirstlane((int)(wg_row0 + (wr * RT + rt) * 16 < a.r_pad)) != 0;
+        row_on[rt] = __builtin_amdgcn_readfirstlane((int)((uint32_t)(wr * RT + rt) < wg_tiles)) != 0;
     });
     const int cc = lane % 16, kq = (lane / 16) * 16;
 
@@ -942,8 +960,8 @@ __device__ __forceinline__ void dense_mfma_areg_body(const MfmaSweepArgs& a, uin
         }
     }
     uint32_t sc[RT][CT][4];
-    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, sc);
-    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, acc, sc);
+    mfma_scales<B, RT, CT, WR, WC>(a, cg, wg_row0, wg_tiles, sc);
+    mfma_store<B, NL, RT, CT, WR, WC>(a, cg, wg_row0, wg_tiles, acc, sc);
 }
 
 template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
@@ -994,10 +1012,8 @@ static uint64_t mfma_args_cfg(const DenseSweep& s, const MfmaTable& t, const int
         a.sw.w[q] = (uint32_t)w;
         w = (w * 256) % m.p;
     }
-    a.xcd_affinity = (8 % a.row_slices == 0) ? 1u : 0u;
-    const uint64_t G = 8 / a.row_slices;
-    const uint64_t blocks = a.xcd_affinity ? 8 * ((a.col_groups + G - 1) / G)
-                                           : a.col_groups * a.row_slices;
+    a.xcd_items = (a.col_groups * a.row_slices + 7) / 8;
+    const uint64_t blocks = 8 * a.xcd_items;
     require(blocks < (1ULL << 31), "MFMA sweep grid too large");
     return blocks;
 }
