@@ -184,6 +184,15 @@ int pgw_apply_forward(pgw_op* op, const uint32_t* d_in, uint32_t* d_out, void* s
  * (cuda/prym_cuda_helpers.h:318-354) without its u32 atomic accumulators. */
 int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_t* d_out,
                    void* stream);
+/* Where the operator's block projections (pgw_gram_upper, the sequence loop,
+ * ungrouped batch and server jobs) run: PGW_METHOD_MFMA on int8 MFMA (digits
+ * of both blocks in registers, int32 digit-shift sums folded to residues
+ * every 256 chunks of 64 rows; b in {4, 8, 16}), PGW_METHOD_PLAIN on the
+ * vector ALUs (64-bit word sums), PGW_METHOD_AUTO the default: MFMA for
+ * operators on the MFMA path.  The reference's projection is the u32 GEMM
+ * dense_gemm_TN_chunked3D_offset_u32 (cuda/prym_cuda_helpers.h:107-137).
+ * Results are identical for every choice. */
+int pgw_operator_set_gram(pgw_op* op, uint32_t method);
 
 /* pgw_gram_upper without an operator and without any allocation: d_out
  * (b(b+1)/2 values) = packed upper triangle of d_x^T d_y mod `prime` for two

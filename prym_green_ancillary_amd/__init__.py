@@ -107,6 +107,7 @@ EXPORTED = [
     "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_bench_stages",
     "pgw_gram_upper_noalloc", "pgw_cyclic_apply_cpu", "pgw_deterministic_block",
     "pgw_wdm_stream_open", "pgw_wdm_stream_terms", "pgw_wdm_stream_finish", "pgw_wdm_stream_destroy",
+    "pgw_operator_set_gram",
 ]
 
 
@@ -202,6 +203,7 @@ def lib():
         "pgw_wdm_stream_terms": (I, [vp, vp, u64]),
         "pgw_wdm_stream_finish": (I, [vp, vp]),
         "pgw_wdm_stream_destroy": (I, [vp]),
+        "pgw_operator_set_gram": (I, [vp, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -468,6 +470,11 @@ class Operator:
 
     def gram_upper(self, d_x: int, d_y: int, d_out: int, stream: int | None = None):
         _check(lib().pgw_gram_upper(self.h, d_x, d_y, d_out, stream))
+
+    def set_gram(self, method: int):
+        """Block projections on int8 MFMA (METHOD_MFMA), the vector ALUs
+        (METHOD_PLAIN) or the default (METHOD_AUTO) (pgw_operator_set_gram)."""
+        _check(lib().pgw_operator_set_gram(self.h, method))
 
     def bench_apply(self, d_a: int, d_b: int, iters: int):
         total, sweep = ctypes.c_double(), ctypes.c_double()

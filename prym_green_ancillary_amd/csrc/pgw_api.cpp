@@ -470,6 +470,7 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             require(method == PGW_METHOD_PLAIN || mfma_ok,
                     "MFMA path needs b in {4, 8, 16}, n <= 32 and k < 16", PGW_ERR_UNSUPPORTED);
             op->method = method;
+            op->gram_mfma = method == PGW_METHOD_MFMA;
             // sweep tables: 0 = D (T, forward), 1 = E (E^T, transpose), 2 = E^T (E, forward),
             // 3 = D^T (T^T, transpose)
             const uint32_t* src[4] = {desc->table_d, desc->table_e, desc->table_e, desc->table_d};
@@ -649,7 +650,23 @@ int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_
         require(op && d_x && d_y && d_out, "null operator or block");
         PGW_HIP(hipSetDevice(op->ctx->device));
         launch_gram(d_x, d_y, op->cols, op->b, 0, d_out, nullptr, op->d_gram, op->m,
-                    pick_stream(op, stream));
+                    pick_stream(op, stream), nullptr, 0, op->gram_mfma);
+    });
+}
+
+int pgw_operator_set_gram(pgw_op* op, uint32_t method) {
+    return guarded([&] {
+        require(op != nullptr, "null operator");
+        if (method == PGW_METHOD_AUTO) {
+            op->gram_mfma = op->method == PGW_METHOD_MFMA;
+        } else if (method == PGW_METHOD_MFMA) {
+            require(op->b == 4 || op->b == 8 || op->b == 16, "MFMA Gram projection needs b in {4, 8, 16}",
+                    PGW_ERR_UNSUPPORTED);
+            op->gram_mfma = true;
+        } else {
+            require(method == PGW_METHOD_PLAIN, "unknown Gram method");
+            op->gram_mfma = false;
+        }
     });
 }
 
@@ -697,7 +714,7 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
             PGW_HIP(hipMemcpy(d_pos, &one, sizeof(uint32_t), hipMemcpyHostToDevice));
         }
         PGW_HIP(hipMemcpyAsync(d_B, h_v, N * b * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        launch_gram(d_B, d_B, N, op->b, 0, d_seq, nullptr, op->d_gram, op->m, st);  // V^T V
+        launch_gram(d_B, d_B, N, op->b, 0, d_seq, nullptr, op->d_gram, op->m, st, nullptr, 0, op->gram_mfma);  // V^T V
         PGW_HIP(hipStreamSynchronize(st));
         if (rounds) {
             // one round (cuprym_cyclic.cu:2439-2449) captured once and replayed:
@@ -706,9 +723,9 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
             PGW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
                 apply_square_impl(op, d_B, d_D, st);
-                launch_gram(d_B, d_D, N, op->b, 1, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 0);
+                launch_gram(d_B, d_D, N, op->b, 1, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 0, op->gram_mfma);
                 apply_square_impl(op, d_D, d_B, st);
-                launch_gram(d_B, d_D, N, op->b, 2, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 2);
+                launch_gram(d_B, d_D, N, op->b, 2, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 2, op->gram_mfma);
                 launch_advance_pos(d_pos, 4, st);
             } catch (...) {
                 hipGraph_t g = nullptr;

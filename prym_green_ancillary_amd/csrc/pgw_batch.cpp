@@ -71,10 +71,10 @@ void enqueue_round(const BatchJob& j, hipStream_t st) {
     const uint64_t ut = (uint64_t)op->b * (op->b + 1) / 2;
     apply_square_impl(op, j.d_B, j.d_D, st);
     launch_gram(j.d_B, j.d_D, op->cols, op->b, 1, j.d_seq, j.d_seq + ut, op->d_gram, op->m, st,
-                j.d_pos, 0);
+                j.d_pos, 0, op->gram_mfma);
     apply_square_impl(op, j.d_D, j.d_B, st);
     launch_gram(j.d_B, j.d_D, op->cols, op->b, 2, j.d_seq, j.d_seq + ut, op->d_gram, op->m, st,
-                j.d_pos, 2);
+                j.d_pos, 2, op->gram_mfma);
     launch_advance_pos(j.d_pos, 4, st);
 }
 
@@ -469,7 +469,7 @@ int pgw_batch_add(pgw_batch* b, pgw_op* op, const uint32_t* h_v, uint64_t max_le
         hipStream_t up = b->upload[u];
         PGW_HIP(hipMemcpyAsync(j.d_B, h_v, N * bb * sizeof(uint32_t), hipMemcpyHostToDevice, up));
         try {
-            launch_gram(j.d_B, j.d_B, N, op->b, 0, j.d_seq, nullptr, op->d_gram, op->m, up);  // V^T V
+            launch_gram(j.d_B, j.d_B, N, op->b, 0, j.d_seq, nullptr, op->d_gram, op->m, up, nullptr, 0, op->gram_mfma);  // V^T V
         } catch (...) {
             (void)hipStreamSynchronize(up);  // the copy still reads h_v
             throw;

@@ -83,9 +83,15 @@ void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64
 // mode 0: out0 = X^T Y; mode 1: out0 = X^T Y, out1 = Y^T Y; mode 2: out0 = X^T Y, out1 = X^T X.
 size_t gram_scratch_bytes(uint32_t b);
 // pos (optional, device): outputs land ut * (*pos + pos_off) entries after out0/out1
+// mfma: the partial sums on int8 MFMA (pgw_gram_mfma.hip, b in {4, 8, 16}),
+// else on the vector ALUs; both write the same partials, the same results.
 void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
                  uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
-                 hipStream_t st, const uint32_t* pos = nullptr, uint32_t pos_off = 0);
+                 hipStream_t st, const uint32_t* pos = nullptr, uint32_t pos_off = 0, bool mfma = false);
+// MFMA partials (pgw_gram_mfma.hip): workgroups for `rows`, and the launch
+unsigned gram_mfma_grid(uint64_t rows, uint32_t b, unsigned max_grid);
+void launch_gram_mfma_partial(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
+                              uint64_t* part, unsigned grid, const ModP& m, hipStream_t st);
 // Allocation-free Gram (mode 0 only, out = X^T Y packed): partials in a
 // fixed device array of gram_noalloc_scratch_bytes() (2 MiB) per GPU.
 // gram_noalloc_setup resolves that array's address on `device` (call it with

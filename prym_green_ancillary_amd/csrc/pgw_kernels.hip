@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -714,18 +716,30 @@ void launch_advance_pos(uint32_t* pos, uint32_t by, hipStream_t st) {
 // the two Gram kernels over at most `max_grid` workgroups (partials in `part`)
 static void gram_launch(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
                         uint32_t* out0, uint32_t* out1, uint64_t* part, unsigned max_grid,
-                        const ModP& m, hipStream_t st, const uint32_t* pos, uint32_t pos_off) {
+                        const ModP& m, hipStream_t st, const uint32_t* pos, uint32_t pos_off,
+                        bool mfma = false) {
     // low words: rows * 2^32 < 2^64 requires rows < 2^32
     require(rows < (1ULL << 32), "Gram projection supports fewer than 2^32 rows");
     require(mode == 0 || out1 != nullptr, "Gram projection of two products needs both outputs");
-    const unsigned grid = (unsigned)std::min<uint64_t>(
-        max_grid, std::max<uint64_t>(1, (rows + kGramRowsPerBlock - 1) / kGramRowsPerBlock));
-    // (the grouped launch, gram_group_blocks, uses the same rule with kGramGrid)
     const uint32_t P = b * (b + 1) / 2;
+    unsigned grid;
+    if (mfma) {
+        // PGW_GRAM_MAX_GRID caps the MFMA grid (more chunks per wave, so the
+        // periodic folds of the accumulators run; a test knob)
+        const char* env = std::getenv("PGW_GRAM_MAX_GRID");
+        const long cv = env ? std::strtol(env, nullptr, 10) : 0;
+        const unsigned cap = cv >= 1 ? (unsigned)std::min<long>(cv, 1L << 20) : 0u;
+        grid = gram_mfma_grid(rows, b, cap ? std::min(cap, max_grid) : max_grid);
+        launch_gram_mfma_partial(x, y, rows, b, mode, part, grid, m, st);
+    } else {
+        grid = (unsigned)std::min<uint64_t>(
+            max_grid, std::max<uint64_t>(1, (rows + kGramRowsPerBlock - 1) / kGramRowsPerBlock));
+        // (the grouped launch, gram_group_blocks, uses the same rule with kGramGrid)
 #define PGW_CALL(BB)                                                                           \
     hipLaunchKernelGGL(k_gram_partial<BB>, dim3(grid), dim3(256), 0, st, x, y, rows, mode, part)
-    PGW_SWITCH_B(b, PGW_CALL)
+        PGW_SWITCH_B(b, PGW_CALL)
 #undef PGW_CALL
+    }
     hipLaunchKernelGGL(k_gram_finish, dim3(P * (mode ? 2 : 1)), dim3(256), 0, st, part, grid, P,
                        out0, out1, m, pos, pos_off);
     PGW_CHECK_LAUNCH("gram");
@@ -733,8 +747,8 @@ static void gram_launch(const uint32_t* x, const uint32_t* y, uint64_t rows, uin
 
 void launch_gram(const uint32_t* x, const uint32_t* y, uint64_t rows, uint32_t b, int mode,
                  uint32_t* out0, uint32_t* out1, uint64_t* scratch, const ModP& m,
-                 hipStream_t st, const uint32_t* pos, uint32_t pos_off) {
-    gram_launch(x, y, rows, b, mode, out0, out1, scratch, kGramGrid, m, st, pos, pos_off);
+                 hipStream_t st, const uint32_t* pos, uint32_t pos_off, bool mfma) {
+    gram_launch(x, y, rows, b, mode, out0, out1, scratch, kGramGrid, m, st, pos, pos_off, mfma);
 }
 
 // ---------------------------------------------------------------------------

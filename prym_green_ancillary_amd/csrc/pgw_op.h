@@ -102,6 +102,9 @@ struct pgw_op {
     uint32_t* d_mid = nullptr;     // mid x b
     uint32_t* d_target = nullptr;  // rows x b
     uint64_t* d_gram = nullptr;
+    // block projection on int8 MFMA (else on the vector ALUs): by default for
+    // operators on the MFMA path, pgw_operator_set_gram to change it
+    bool gram_mfma = false;
     // synthetic
     uint32_t n = 0, k = 0, M = 0, Mp = 0;
     uint64_t u_src = 0, u_tgt = 0;
