@@ -182,9 +182,9 @@ def sector_macs(spec, b):
 def bench_batch(args):
     """64 uneven sectors (seed 3): `--rounds` sequence rounds of every sector,
     all in one batch (grouped launches, graph-replayed rounds) against the same
-    batch without grouping, the sectors run one at a time, and the batch with
-    tile configurations autotuned per sector shape.  Results of the four runs
-    must be identical."""
+    batch without grouping, with its grouped sweeps as persistent launches,
+    the sectors run one at a time, and the batch with tile configurations
+    autotuned per sector shape.  Results of the five runs must be identical."""
     import prym_green_ancillary_amd as pg
     from prym_green_ancillary_amd import workloads as wl
 
@@ -203,10 +203,10 @@ def bench_batch(args):
     applies = 2 * args.rounds * len(jobs)
     macs = sum(2 * args.rounds * sector_macs(j.spec, j.block) for j in jobs)
 
-    def run(sets, grouped, autotune=False):
+    def run(sets, grouped, autotune=False, persistent=False):
         out, secs, add_secs = {}, 0.0, 0.0
         for g in sets:
-            batch = pg.Batch(ctx, use_graph=True, grouped=grouped, autotune=autotune)
+            batch = pg.Batch(ctx, use_graph=True, grouped=grouped, autotune=autotune, persistent=persistent)
             t0 = time.perf_counter()
             ids = [(i, batch.add(ops[i], vs[i], max_len)) for i in g]
             add_secs += time.perf_counter() - t0
@@ -225,11 +225,12 @@ def bench_batch(args):
     everything = [list(range(len(jobs)))]
     together, t_batch, _, _ = run(everything, True)
     ungrouped, t_ungrouped, _, _ = run(everything, False)
+    persistent, t_persistent, _, _ = run(everything, True, persistent=True)
     alone, t_alone, _, _ = run([[i] for i in range(len(jobs))], False)
     # last: the autotuner keeps its choices in the operators
     tuned, t_tuned, t_tune_adds, tuned_groups = run(everything, True, autotune=True)
     for i in range(len(jobs)):
-        for other in (together, ungrouped, tuned):
+        for other in (together, ungrouped, persistent, tuned):
             if not ((other[i][0] == alone[i][0]).all() and (other[i][1] == alone[i][1]).all()):
                 print(f"sector {i}: batched and lone results differ: no number reported",
                       file=sys.stderr)
@@ -251,6 +252,7 @@ def bench_batch(args):
         "gf_macs_per_s": float(f"{macs / t_batch:.4g}"),
         "one_at_a_time_block_applies_per_s": round(applies / t_alone, 3),
         "ungrouped_block_applies_per_s": round(applies / t_ungrouped, 3),
+        "persistent_sweeps_block_applies_per_s": round(applies / t_persistent, 3),
         "autotuned": {"block_applies_per_s": round(applies / t_tuned, 3),
                       "add_s_with_tuning": round(t_tune_adds, 2),
                       "groups_and_grouped_jobs_at_end": list(tuned_groups)},

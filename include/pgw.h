@@ -304,8 +304,11 @@ typedef struct pgw_batch pgw_batch;
 enum pgw_batch_flags {
     PGW_BATCH_GRAPH = 1,    /* replay a captured graph of the round (else launch every replay) */
     PGW_BATCH_GROUPED = 2,  /* one launch per stage for each group of compatible jobs */
-    PGW_BATCH_AUTOTUNE = 4  /* time tile configurations 7-9 once per operator shape (n, k, M, M',
-                               b, limbs, fold) and give every job of that shape the fastest */
+    PGW_BATCH_AUTOTUNE = 4,  /* time tile configurations 7-9 once per operator shape (n, k, M, M',
+                                b, limbs, fold) and give every job of that shape the fastest */
+    PGW_BATCH_PERSISTENT = 8  /* grouped sweeps as persistent launches: two workgroups per CU walk
+                                 every tile of the group (default: one workgroup per tile, which
+                                 measured faster on the 64-sector mix) */
 };
 
 int pgw_batch_create(pgw_ctx* ctx, uint32_t flags, pgw_batch** out);

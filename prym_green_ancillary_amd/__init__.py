@@ -542,14 +542,14 @@ class Batch:
     of every job per replay on one stream; compatible MFMA jobs share one
     launch per stage (grouped), a round replays as a captured graph."""
 
-    GRAPH, GROUPED, AUTOTUNE = 1, 2, 4
+    GRAPH, GROUPED, AUTOTUNE, PERSISTENT = 1, 2, 4, 8
 
     def __init__(self, ctx: Context, use_graph: bool = True, grouped: bool = True,
-                 autotune: bool = False):
+                 autotune: bool = False, persistent: bool = False):
         self.ctx = ctx
         h = ctypes.c_void_p()
         flags = ((self.GRAPH if use_graph else 0) | (self.GROUPED if grouped else 0)
-                 | (self.AUTOTUNE if autotune else 0))
+                 | (self.AUTOTUNE if autotune else 0) | (self.PERSISTENT if persistent else 0))
         _check(lib().pgw_batch_create(ctx.h, flags, ctypes.byref(h)))
         self.h = h
         self.jobs = {}

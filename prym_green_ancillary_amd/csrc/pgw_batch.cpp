@@ -101,6 +101,7 @@ struct GroupArrays {
 struct Group {
     uint32_t b = 0, nl = 0, rounds = 0;
     bool m31 = false;
+    bool persistent = false;  // PGW_BATCH_PERSISTENT: the sweeps as persistent launches
     int cfg[4] = {8, 8, 8, 8};
     std::vector<uint64_t> ids;
     std::vector<void*> allocs;
@@ -226,9 +227,9 @@ struct Group {
         const uint32_t n = (uint32_t)ids.size();
         const GroupArrays<LimbJob>& src = from_B ? split_B : split_D;
         launch_limb_split_group(src.d_jobs, src.d_first, n, src.blocks, b, nl, st);
-        mfma_group_launch(cfg[0], b, nl, true, sweep[0].d_jobs, sweep[0].d_first, n, sweep[0].blocks, st);
+        mfma_group_launch(cfg[0], b, nl, true, sweep[0].d_jobs, sweep[0].d_first, n, sweep[0].blocks, persistent, st);
         launch_limb_split_group(split_mid.d_jobs, split_mid.d_first, n, split_mid.blocks, b, nl, st);
-        mfma_group_launch(cfg[1], b, nl, false, sweep[1].d_jobs, sweep[1].d_first, n, sweep[1].blocks, st);
+        mfma_group_launch(cfg[1], b, nl, false, sweep[1].d_jobs, sweep[1].d_first, n, sweep[1].blocks, persistent, st);
         // P^T Dr P: rounds 0..R-2 forward, the fused last round, R-2..0 transposed
         for (uint32_t r = 0; r + 1 < rounds; ++r)
             launch_rowmix_group(mix.d_jobs, mix.d_first, n, mix.blocks, b, 0, m31, r, st);
@@ -236,10 +237,10 @@ struct Group {
         for (uint32_t r = rounds - 1; r-- > 0;)
             launch_rowmix_group(mix.d_jobs, mix.d_first, n, mix.blocks, b, 1, m31, r, st);
         launch_limb_split_group(split_tgt.d_jobs, split_tgt.d_first, n, split_tgt.blocks, b, nl, st);
-        mfma_group_launch(cfg[2], b, nl, true, sweep[2].d_jobs, sweep[2].d_first, n, sweep[2].blocks, st);
+        mfma_group_launch(cfg[2], b, nl, true, sweep[2].d_jobs, sweep[2].d_first, n, sweep[2].blocks, persistent, st);
         launch_limb_split_group(split_mid.d_jobs, split_mid.d_first, n, split_mid.blocks, b, nl, st);
         const GroupArrays<MfmaSweepArgs>& s3 = from_B ? sweep3_D : sweep3_B;
-        mfma_group_launch(cfg[3], b, nl, false, s3.d_jobs, s3.d_first, n, s3.blocks, st);
+        mfma_group_launch(cfg[3], b, nl, false, s3.d_jobs, s3.d_first, n, s3.blocks, persistent, st);
     }
 
     void round(hipStream_t st) const {
@@ -310,6 +311,7 @@ struct pgw_batch {
         std::lock_guard<std::mutex> cl(ctx->capture_mtx);
         for (auto& kv : classes) {
             groups.emplace_back();
+            groups.back().persistent = (flags & PGW_BATCH_PERSISTENT) != 0;
             try {
                 groups.back().build(kv.second);
             } catch (...) {
@@ -409,7 +411,8 @@ int pgw_batch_create(pgw_ctx* ctx, uint32_t flags, pgw_batch** out) {
     pgw_batch* b = nullptr;
     int rc = guarded([&] {
         require(ctx && out, "null context or output pointer");
-        require((flags & ~(uint32_t)(PGW_BATCH_GRAPH | PGW_BATCH_GROUPED | PGW_BATCH_AUTOTUNE)) == 0,
+        require((flags & ~(uint32_t)(PGW_BATCH_GRAPH | PGW_BATCH_GROUPED | PGW_BATCH_AUTOTUNE |
+                                     PGW_BATCH_PERSISTENT)) == 0,
                 "unknown batch flags");
         PGW_HIP(hipSetDevice(ctx->device));
         b = new pgw_batch;

@@ -133,13 +133,22 @@ void launch_limb_split(const uint32_t* in, const uint32_t* scale, int8_t* lp, ui
 // first[lo] <= blockIdx.x < first[lo + 1].  Ranges are multiples of 8 blocks
 // for the MFMA sweeps, so that a block's XCD (hardware block id mod 8) is its
 // local id mod 8.
-__device__ __forceinline__ uint32_t group_job(const uint32_t* __restrict__ first, uint32_t njobs) {
+// Grid of a persistent launch on the current device: per_cu workgroups per
+// CU, a multiple of 8 (the XCD count).
+uint32_t persistent_grid(uint32_t per_cu);
+
+// the job that owns block `blk` of a grouped grid (first[j] <= blk < first[j + 1])
+__device__ __forceinline__ uint32_t group_job_at(const uint32_t* __restrict__ first, uint32_t njobs,
+                                                 uint32_t blk) {
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
-        if (first[mid] <= blockIdx.x) lo = mid; else hi = mid;
+        if (first[mid] <= blk) lo = mid; else hi = mid;
     }
     return lo;
+}
+__device__ __forceinline__ uint32_t group_job(const uint32_t* __restrict__ first, uint32_t njobs) {
+    return group_job_at(first, njobs, blockIdx.x);
 }
 
 struct LimbJob {
@@ -195,7 +204,7 @@ uint64_t mfma_group_prepare(int cfg, uint32_t b, uint32_t nl, const DenseSweep& 
                             const int8_t* lp, uint32_t* out, const uint32_t* scale, const ModP& m,
                             MfmaSweepArgs& a);
 void mfma_group_launch(int cfg, uint32_t b, uint32_t nl, bool fwd, const MfmaSweepArgs* jobs,
-                       const uint32_t* first, uint32_t njobs, uint64_t blocks, hipStream_t st);
+                       const uint32_t* first, uint32_t njobs, uint64_t blocks, bool persistent, hipStream_t st);
 // cfg selects a tile configuration in [0, mfma_config_count())
 int mfma_config_count();
 void launch_dense_mfma(const DenseSweep& s, const MfmaTable& t, const int8_t* lp, uint32_t* out,

@@ -19,9 +19,9 @@ uint64_t mfma_group_prepare_b16(int cfg, uint32_t nl, const DenseSweep& s, const
 }
 
 void mfma_group_launch_b16(int cfg, uint32_t nl, bool fwd, const MfmaSweepArgs* jobs,
-                           const uint32_t* first, uint32_t njobs, uint64_t blocks, hipStream_t st) {
-    if (nl == 4) mfma_group_launch_t<16, 4>(cfg, fwd, jobs, first, njobs, blocks, st);
-    else mfma_group_launch_t<16, 2>(cfg, fwd, jobs, first, njobs, blocks, st);
+                           const uint32_t* first, uint32_t njobs, uint64_t blocks, bool persistent, hipStream_t st) {
+    if (nl == 4) mfma_group_launch_t<16, 4>(cfg, fwd, jobs, first, njobs, blocks, persistent, st);
+    else mfma_group_launch_t<16, 2>(cfg, fwd, jobs, first, njobs, blocks, persistent, st);
 }
 
 }  // namespace pgw

@@ -2,6 +2,8 @@
 // table contraction: table digits, limb split, launch dispatch by block width.
 #include "pgw_mfma_sweep.h"
 
+#include <cstdlib>
+
 namespace pgw {
 
 static inline uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
@@ -212,6 +214,13 @@ void launch_limb_split_group(const LimbJob* jobs, const uint32_t* first, uint32_
 // ---------------------------------------------------------------------------
 int mfma_config_count() { return 10; }
 
+uint32_t persistent_grid(uint32_t per_cu) {
+    int dev = 0, cus = 0;
+    PGW_HIP(hipGetDevice(&dev));
+    PGW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return std::max<uint32_t>(8, (uint32_t)cus * per_cu / 8 * 8);
+}
+
 void launch_dense_mfma_b4(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
                           uint32_t*, const uint32_t*, const ModP&, hipStream_t);
 void launch_dense_mfma_b8(int, uint32_t, const DenseSweep&, const MfmaTable&, const int8_t*,
@@ -224,7 +233,7 @@ void launch_dense_mfma_b16(int, uint32_t, const DenseSweep&, const MfmaTable&, c
                                       const int8_t*, uint32_t*, const uint32_t*, const ModP&,       \
                                       MfmaSweepArgs&);                                              \
     void mfma_group_launch_b##BB(int, uint32_t, bool, const MfmaSweepArgs*, const uint32_t*,        \
-                                 uint32_t, uint64_t, hipStream_t);
+                                 uint32_t, uint64_t, bool, hipStream_t);
 PGW_GROUP_DECL(4)
 PGW_GROUP_DECL(8)
 PGW_GROUP_DECL(16)
@@ -240,10 +249,10 @@ uint64_t mfma_group_prepare(int cfg, uint32_t b, uint32_t nl, const DenseSweep& 
 }
 
 void mfma_group_launch(int cfg, uint32_t b, uint32_t nl, bool fwd, const MfmaSweepArgs* jobs,
-                       const uint32_t* first, uint32_t njobs, uint64_t blocks, hipStream_t st) {
-    if (b == 16) mfma_group_launch_b16(cfg, nl, fwd, jobs, first, njobs, blocks, st);
-    else if (b == 8) mfma_group_launch_b8(cfg, nl, fwd, jobs, first, njobs, blocks, st);
-    else if (b == 4) mfma_group_launch_b4(cfg, nl, fwd, jobs, first, njobs, blocks, st);
+                       const uint32_t* first, uint32_t njobs, uint64_t blocks, bool persistent, hipStream_t st) {
+    if (b == 16) mfma_group_launch_b16(cfg, nl, fwd, jobs, first, njobs, blocks, persistent, st);
+    else if (b == 8) mfma_group_launch_b8(cfg, nl, fwd, jobs, first, njobs, blocks, persistent, st);
+    else if (b == 4) mfma_group_launch_b4(cfg, nl, fwd, jobs, first, njobs, blocks, persistent, st);
     else throw Error(PGW_ERR_UNSUPPORTED, "MFMA path supports b in {4, 8, 16}");
 }
 

@@ -970,13 +970,22 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg(MfmaSweepAr
 }
 
 // many sectors' sweeps in one launch (same B, NL, tile and direction): the
-// arguments of job j at jobs[j], its blocks [first[j], first[j+1])
+// arguments of job j at jobs[j], its tiles [first[j], first[j+1]) of
+// `total`.  Workgroup w walks tiles w, w + grid, w + 2 grid, ...: with one
+// workgroup per tile that is one tile; as a persistent launch (a grid of a
+// few workgroups per CU) it is every grid-th tile.  Every first[j] and the
+// grid are multiples of 8, so a tile runs on the XCD its job-local index maps
+// it to.  A barrier opens each tile: the waves of the previous one may still
+// read the LDS stages the next one fills.
 template <int B, int NL, int RT, int CT, int WR, int WC, int STG, bool FWD>
 __global__ __launch_bounds__(64 * WR * WC, 2) void k_dense_mfma_areg_group(
-    const MfmaSweepArgs* __restrict__ jobs, const uint32_t* __restrict__ first, uint32_t njobs) {
-    const uint32_t j = group_job(first, njobs);
-    const MfmaSweepArgs a = jobs[j];
-    dense_mfma_areg_body<B, NL, RT, CT, WR, WC, STG, FWD>(a, blockIdx.x - first[j]);
+    const MfmaSweepArgs* __restrict__ jobs, const uint32_t* __restrict__ first, uint32_t njobs, uint32_t total) {
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        __syncthreads();
+        const uint32_t j = group_job_at(first, njobs, t);
+        const MfmaSweepArgs a = jobs[j];
+        dense_mfma_areg_body<B, NL, RT, CT, WR, WC, STG, FWD>(a, t - first[j]);
+    }
 }
 
 // Arguments and grid of one sweep launch of tile configuration (RT, CT, WR, WC).
@@ -1056,12 +1065,16 @@ static uint64_t mfma_group_prepare_t(int cfg, const DenseSweep& s, const MfmaTab
 
 template <int B, int NL>
 static void mfma_group_launch_t(int cfg, bool fwd, const MfmaSweepArgs* jobs, const uint32_t* first,
-                                uint32_t njobs, uint64_t blocks, hipStream_t st) {
+                                uint32_t njobs, uint64_t blocks, bool persistent, hipStream_t st) {
     constexpr int CT = NL == 4 ? 2 : 4;
-    require(blocks < (1ULL << 31), "grouped MFMA sweep grid too large");
+    require(blocks < (1ULL << 31) && blocks % 8 == 0, "grouped MFMA sweep grid too large or unaligned");
+    // one workgroup per tile, or persistent: two workgroups per CU
+    // (configuration 7's eight waves: one) walking all the group's tiles
+    const unsigned grid = persistent ? (unsigned)std::min<uint64_t>(blocks, persistent_grid(cfg == 7 ? 1 : 2))
+                                     : (unsigned)blocks;
 #define PGW_MG(WR, STG, F)                                                                          \
-    hipLaunchKernelGGL((k_dense_mfma_areg_group<B, NL, 2, CT, WR, 1, STG, F>), dim3((unsigned)blocks), \
-                       dim3(64 * WR), 0, st, jobs, first, njobs)
+    hipLaunchKernelGGL((k_dense_mfma_areg_group<B, NL, 2, CT, WR, 1, STG, F>), dim3(grid),           \
+                       dim3(64 * WR), 0, st, jobs, first, njobs, (uint32_t)blocks)
     switch (cfg) {
         case 7: if (fwd) PGW_MG(8, 2, true); else PGW_MG(8, 2, false); break;
         case 8: if (fwd) PGW_MG(4, 2, true); else PGW_MG(4, 2, false); break;

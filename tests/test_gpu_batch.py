@@ -45,15 +45,16 @@ def make_op(ctx, job, method, seed=3):
     return op, (T, E, dr, dc, v, pairs)
 
 
-@pytest.mark.parametrize("use_graph,grouped,autotune", [(True, True, False), (True, False, False),
-                                                       (False, True, False), (False, False, False),
-                                                       (True, True, True)])
-def test_batch_rounds_match_single_runs_and_oracle(ctx, use_graph, grouped, autotune):
+@pytest.mark.parametrize("use_graph,grouped,autotune,persistent", [
+    (True, True, False, False), (True, False, False, False), (False, True, False, False),
+    (False, False, False, False), (True, True, True, False), (True, True, False, True),
+    (False, True, False, True)])
+def test_batch_rounds_match_single_runs_and_oracle(ctx, use_graph, grouped, autotune, persistent):
     # MFMA jobs of one block width share launches (grouped); plain jobs run alone;
     # with autotune every MFMA shape gets its fastest of tile configurations 7-9
     jobs = small_mix()
     lengths = [5 + 7 * i for i in range(len(jobs))]  # uneven lengths: jobs finish at different steps
-    batch = pg.Batch(ctx, use_graph=use_graph, grouped=grouped, autotune=autotune)
+    batch = pg.Batch(ctx, use_graph=use_graph, grouped=grouped, autotune=autotune, persistent=persistent)
     ops, ids, state = [], [], []
     for j, L in zip(jobs, lengths):
         method = pg.METHOD_MFMA if j.index % 2 == 0 else pg.METHOD_PLAIN
@@ -116,14 +117,16 @@ def test_batch_mixes_cyclic_and_synthetic(ctx, g12):
             assert (seq == oseq).all() and (curv == ocurv).all()
 
 
-def test_batch_c3_mix_rounds_match_lone_runs_and_oracle(ctx):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_c3_mix_rounds_match_lone_runs_and_oracle(ctx, persistent):
     """The batching workload itself: 64 uneven sectors (seed 3: n 12..22,
     k 3..6, M 20..300, p = 65521, b 8 or 16, N 14300..4.79M), two rounds
-    each, grouped and graph-replayed; every job equals its lone run, the
-    smallest sectors equal the oracle."""
+    each, grouped and graph-replayed (the grouped sweeps with one workgroup
+    per tile, or persistent); every job equals its lone run, the smallest
+    sectors equal the oracle."""
     jobs = wl.uneven_sectors(seed=3, count=64)
     max_len = 9  # two rounds
-    batch = pg.Batch(ctx, use_graph=True, grouped=True)
+    batch = pg.Batch(ctx, use_graph=True, grouped=True, persistent=persistent)
     entries = []
     for j in jobs:
         dr, dc, v, pairs = wl.sector_state(j, 3)
