@@ -14,8 +14,17 @@ using namespace pgw;
 
 namespace {
 
+uint32_t bits_for(uint64_t v) {  // bits of the largest value below v (at least 1)
+    uint32_t b = 1;
+    while (b < 64 && (v - 1) >> b) ++b;
+    return b;
+}
+
 // A sparse map in its device (sliced-ELL) form, owned by the operator's arena.
-SellDevice upload_sell(DeviceArena& A, const CsrHost& csr) {
+// When a column index and a coefficient (< p) fit in 32 bits together, every
+// entry is one packed u32 (column | coefficient << column bits): half the
+// bytes the map streams per apply.  The paper instances up to g26 qualify.
+SellDevice upload_sell(DeviceArena& A, const CsrHost& csr, uint32_t p) {
     const SellHost h = csr_to_sell(csr);
     SellDevice d;
     d.rows = h.rows;
@@ -23,7 +32,23 @@ SellDevice upload_sell(DeviceArena& A, const CsrHost& csr) {
     d.nnz = h.nnz;
     d.padded = h.slice.back();
     d.slice = A.upload(h.slice);
-    d.ent = A.upload(h.ent);
+    const uint32_t cb = bits_for(std::max<uint64_t>(h.cols, 2));
+    // PGW_SELL_PACK=0 keeps the pairs (a test knob for the unpacked kernels)
+    const char* knob = std::getenv("PGW_SELL_PACK");
+    const bool allow = !(knob && knob[0] == '0');
+    if (allow && cb + bits_for(p) <= 32) {
+        std::vector<uint32_t> packed(d.padded);
+        for (uint64_t q = 0; q < d.padded; ++q) {
+            const uint32_t col = h.ent[2 * q], coef = h.ent[2 * q + 1];
+            require((uint64_t)col < (1ULL << cb) && (uint64_t)coef < (1ULL << (32 - cb)),
+                    "sparse map entry outside its packed range");
+            packed[q] = col | (coef << cb);
+        }
+        d.ent = A.upload(packed);
+        d.pack_bits = cb;
+    } else {
+        d.ent = A.upload(h.ent);
+    }
     return d;
 }
 
@@ -91,7 +116,8 @@ double rowmix_bytes(const pgw_op* op) {
 }
 double sell_bytes(const SellDevice& M, uint32_t b) {
     const uint64_t slices = (M.rows + kSellHeight - 1) / kSellHeight;
-    return (double)M.padded * 8 + (slices + 1) * 8.0 + (double)M.cols * b * 4 + (double)M.rows * b * 4;
+    return (double)M.padded * (M.pack_bits ? 4 : 8) + (slices + 1) * 8.0 + (double)M.cols * b * 4 +
+           (double)M.rows * b * 4;
 }
 
 void timed_limb_split(pgw_op* op, const uint32_t* in, const uint32_t* scale, int8_t* lp, uint64_t v,
@@ -572,13 +598,13 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             layout = CyclicLayout();
             op->y_rows = maps.m1.rows;
             op->mid = op->y_rows;
-            op->m1 = upload_sell(A, maps.m1);
-            op->m1t = upload_sell(A, csr_transpose(maps.m1));
+            op->m1 = upload_sell(A, maps.m1, op->p);
+            op->m1t = upload_sell(A, csr_transpose(maps.m1), op->p);
             maps.m1 = CsrHost();
-            op->m2 = upload_sell(A, maps.m2);
-            op->m2t = upload_sell(A, csr_transpose(maps.m2));
+            op->m2 = upload_sell(A, maps.m2, op->p);
+            op->m2t = upload_sell(A, csr_transpose(maps.m2), op->p);
             maps.m2 = CsrHost();
-            op->m3 = upload_sell(A, maps.m3);
+            op->m3 = upload_sell(A, maps.m3, op->p);
             op->d_y = A.alloc<uint32_t>(op->y_rows * b);
             op->d_t2 = A.alloc<uint32_t>(op->rows * b);
         }

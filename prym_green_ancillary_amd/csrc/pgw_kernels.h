@@ -64,7 +64,11 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 struct SellDevice {
     uint64_t rows = 0, cols = 0, nnz = 0, padded = 0;  // padded: stored entries
     const uint64_t* slice = nullptr;
+    // pack_bits = 0: (column, coefficient) u32 pairs; otherwise one u32 per
+    // entry, column in the low pack_bits bits and the coefficient above them
+    // (when the column count and p fit together in 32 bits)
     const uint32_t* ent = nullptr;
+    uint32_t pack_bits = 0;
 };
 // out (M.rows x b) = M in (M.cols x b) mod p; out must not overlap in
 void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,

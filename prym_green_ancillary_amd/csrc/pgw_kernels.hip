@@ -188,11 +188,11 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 // SMALLP (p < 2^16): products < 2^32, summed in u64; otherwise products
 // < 2^62 summed as 64-bit low/high word sums.  Exact for rows of any length
 // below 2^32.
-template <int B, int CW, int LANES, bool SMALLP>
+template <int B, int CW, int LANES, bool SMALLP, bool PACKED>
 __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ slice,
-                                                   const uint2* __restrict__ ent, uint64_t rows,
+                                                   const uint32_t* __restrict__ ent, uint64_t rows,
                                                    const uint32_t* __restrict__ in,
-                                                   uint32_t* __restrict__ out, ModP m) {
+                                                   uint32_t* __restrict__ out, ModP m, uint32_t pack_bits) {
     constexpr int G = B / CW;
     constexpr int T = G * LANES;  // threads per row; LANES > 1 only when T divides 64
     const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -205,9 +205,16 @@ __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ 
     if (valid) {
         const uint64_t sl = row / kSellHeight;
         const uint64_t b0 = slice[sl], width = (slice[sl + 1] - b0) / kSellHeight;
-        const uint2* e = ent + b0 + row % kSellHeight;
+        const uint64_t e0 = b0 + row % kSellHeight;
+        const uint32_t col_mask = PACKED ? (1u << pack_bits) - 1u : 0u;
         for (uint64_t k = lane; k < width; k += LANES) {
-            const uint2 c = e[k * kSellHeight];
+            uint2 c;
+            if constexpr (PACKED) {
+                const uint32_t w = ent[e0 + k * kSellHeight];
+                c = make_uint2(w & col_mask, w >> pack_bits);
+            } else {
+                c = reinterpret_cast<const uint2*>(ent)[e0 + k * kSellHeight];
+            }
             const uint32_t* xr = in + (uint64_t)c.x * B + g * CW;
             uint32_t xv[CW];
             if constexpr (CW == 4) {
@@ -251,11 +258,15 @@ static void sell_launch(const SellDevice& M, const uint32_t* in, uint32_t* out, 
                         hipStream_t st) {
     const uint64_t threads = M.rows * (B / CW) * LANES;
     const dim3 grid(grid_for(threads)), blk(256);
-    const uint2* ent = reinterpret_cast<const uint2*>(M.ent);
-    if (m.p < 65536u)
-        hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, true>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
-    else
-        hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, false>), grid, blk, 0, st, M.slice, ent, M.rows, in, out, m);
+#define PGW_SELL(SP, PK)                                                                                   \
+    hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, SP, PK>), grid, blk, 0, st, M.slice, M.ent, M.rows, in, out, \
+                       m, M.pack_bits)
+    if (m.p < 65536u) {
+        if (M.pack_bits) PGW_SELL(true, true); else PGW_SELL(true, false);
+    } else {
+        if (M.pack_bits) PGW_SELL(false, true); else PGW_SELL(false, false);
+    }
+#undef PGW_SELL
 }
 
 // lanes per row: enough threads to fill the chip (about 2^20) when rows are

@@ -72,8 +72,13 @@ def gpu_apply(torch, op, x):
 
 # ------------------------------------------------------------ cyclic (g12) --
 
-@pytest.mark.parametrize("b", [4, 3])
-def test_g12_apply_square_all_sectors(torch_cuda, ctx, g12, b):
+@pytest.mark.parametrize("b,packed", [(4, True), (3, True), (4, False)])
+def test_g12_apply_square_all_sectors(torch_cuda, ctx, g12, b, packed, monkeypatch):
+    # packed: sparse-map entries as one u32 (column | coefficient << column
+    # bits), which g12's sizes allow; PGW_SELL_PACK=0 keeps (column,
+    # coefficient) pairs, the form of instances too large to pack
+    if not packed:
+        monkeypatch.setenv("PGW_SELL_PACK", "0")
     p = g12["modulus"]
     for sector in range(g12["cyclic"]["order"]):
         _log(f"g12 sector {sector} b={b}")
