@@ -326,6 +326,15 @@ int pgw_batch_poll(pgw_batch* b, uint64_t job, int* done);
  * forgets the job. */
 int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_curv,
                      uint64_t* len);
+/* The terms of a job that are final while it still runs: *done = 1 + 4 *
+ * (rounds of replays already synchronised), at most L, and with h_seq the
+ * rows [first, *done) are copied to h_seq + first * b(b+1)/2 (h_seq sized
+ * for all L rows; NULL only reports *done).  The copy runs on a stream of
+ * its own, so a pgw_batch_step in progress is not waited for.  It must not
+ * overlap pgw_batch_result, _cancel or _destroy of the same job.  No
+ * counterpart in the reference; a serving loop feeds these terms to the rank
+ * stage during the job. */
+int pgw_batch_terms(pgw_batch* b, uint64_t job, uint64_t first, uint32_t* h_seq, uint64_t* done);
 /* Drops a job (finished or not) and frees its device state. */
 int pgw_batch_cancel(pgw_batch* b, uint64_t job);
 /* Number of graph captures so far (instrumentation). */

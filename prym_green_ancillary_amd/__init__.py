@@ -107,7 +107,7 @@ EXPORTED = [
     "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_bench_stages",
     "pgw_gram_upper_noalloc", "pgw_cyclic_apply_cpu", "pgw_deterministic_block",
     "pgw_wdm_stream_open", "pgw_wdm_stream_terms", "pgw_wdm_stream_finish", "pgw_wdm_stream_destroy",
-    "pgw_operator_set_gram",
+    "pgw_operator_set_gram", "pgw_batch_terms",
 ]
 
 
@@ -204,6 +204,7 @@ def lib():
         "pgw_wdm_stream_finish": (I, [vp, vp]),
         "pgw_wdm_stream_destroy": (I, [vp]),
         "pgw_operator_set_gram": (I, [vp, u32]),
+        "pgw_batch_terms": (I, [vp, u64, u64, vp, ctypes.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -595,6 +596,15 @@ class Batch:
         while self.step(max_replays or (1 << 30)):
             pass
         return {j: self.result(j) for j in list(self.jobs)}
+
+    def terms(self, job: int, first: int = 0, seq=None) -> int:
+        """Terms of a running job that are already final (pgw_batch_terms);
+        with `seq` (a uint32 (L, b(b+1)/2) array) rows [first, done) are
+        copied into it."""
+        done = ctypes.c_uint64()
+        _check(lib().pgw_batch_terms(self.h, job, first, None if seq is None else _ptr(seq),
+                                     ctypes.byref(done)))
+        return done.value
 
     def graph_builds(self) -> int:
         n = ctypes.c_uint64()

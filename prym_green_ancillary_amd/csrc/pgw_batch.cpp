@@ -541,6 +541,34 @@ int pgw_batch_result(pgw_batch* b, uint64_t job, uint32_t* h_seq, uint32_t* h_cu
     });
 }
 
+int pgw_batch_terms(pgw_batch* b, uint64_t job, uint64_t first, uint32_t* h_seq, uint64_t* done) {
+    return guarded([&] {
+        require(b && done, "null batch or output");
+        uint32_t* d_seq = nullptr;
+        uint64_t ut = 0, upto = 0;
+        {
+            std::lock_guard<std::mutex> lk(b->mtx);
+            auto it = b->jobs.find(job);
+            require(it != b->jobs.end(), "unknown batch job");
+            const BatchJob& j = it->second;
+            ut = (uint64_t)j.op->b * (j.op->b + 1) / 2;
+            // rounds_done counts replays whose stream was synchronised
+            upto = std::min<uint64_t>(j.L, 1 + 4 * j.rounds_done);
+            d_seq = j.d_seq;
+        }
+        *done = upto;
+        if (!h_seq || first >= upto) return;
+        // on an upload stream, so that a step in progress is not waited for
+        const unsigned u = b->next_upload.fetch_add(1) % kUploadStreams;
+        std::lock_guard<std::mutex> ul(b->upload_mtx[u]);
+        std::lock_guard<std::mutex> cl(b->ctx->capture_mtx);
+        PGW_HIP(hipSetDevice(b->ctx->device));
+        PGW_HIP(hipMemcpyAsync(h_seq + first * ut, d_seq + first * ut, (upto - first) * ut * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, b->upload[u]));
+        PGW_HIP(hipStreamSynchronize(b->upload[u]));
+    });
+}
+
 int pgw_batch_cancel(pgw_batch* b, uint64_t job) {
     return guarded([&] {
         require(b != nullptr, "null batch");
@@ -679,7 +707,18 @@ struct ServerJob {
     std::vector<uint32_t> gen;
     uint64_t gen_terms = 0;
     double rank_s = 0;
+    // the rank stage fed while the job runs (no generator asked for): the
+    // consumer's state, the terms it was given, and the feed in progress
+    pgw_rank* rk = nullptr;
+    uint64_t fed = 0;
+    int feed = 0;  // 0 none, 1 queued, 2 running
 };
+
+// Terms a running job must have gained before the rank stage is fed again:
+// at least this many, and at least as many as it was given so far, so the
+// incremental steps stay few (about log2(L / kFeedMin)) and the consumer's
+// total work stays within about twice one pass over the whole sequence.
+constexpr uint64_t kFeedMin = 512;
 
 // Device bytes of a job: operator (tables, digit planes, intermediate blocks,
 // preconditioners, row-mix plan, Gram scratch) plus its sequence state.
@@ -717,8 +756,9 @@ struct pgw_server {
     uint32_t max_active = 0, replays_per_step = 1;
     pgw_batch* batch = nullptr;
     std::mutex mtx;
-    std::condition_variable cv_io, cv_step, cv_done;
+    std::condition_variable cv_io, cv_step, cv_done, cv_feed;
     std::deque<uint64_t> queue;                              // submitted, not yet admitted
+    std::deque<uint64_t> feeds;                              // running jobs with terms for the rank stage
     std::vector<uint64_t> running;                           // in the batch
     std::deque<std::pair<uint64_t, std::string>> finished;   // left the batch (error text if failed)
     std::map<uint64_t, std::unique_ptr<ServerJob>> jobs;
@@ -754,16 +794,21 @@ struct pgw_server {
         (void)hipSetDevice(ctx->device);
         for (;;) {
             ServerJob* admit = nullptr;
+            ServerJob* feed = nullptr;
             std::pair<uint64_t, std::string> fin{0, ""};
             {
                 std::unique_lock<std::mutex> lk(mtx);
                 cv_io.wait(lk, [&] {
-                    return !finished.empty() || can_admit() ||
+                    return !finished.empty() || can_admit() || !feeds.empty() ||
                            (stop && queue.empty() && running.empty() && starting == 0);
                 });
                 if (!finished.empty()) {
                     fin = finished.front();
                     finished.pop_front();
+                } else if (!feeds.empty() && !can_admit()) {
+                    feed = jobs.at(feeds.front()).get();
+                    feeds.pop_front();
+                    feed->feed = 2;
                 } else if (can_admit()) {
                     admit = jobs.at(queue.front()).get();
                     queue.pop_front();
@@ -778,6 +823,7 @@ struct pgw_server {
             }
             if (fin.first) collect(fin.first, fin.second);
             if (admit) start(*admit);
+            if (feed) feed_rank(*feed);
         }
     }
 
@@ -815,6 +861,14 @@ struct pgw_server {
         in_use = in_use - j.charge + actual;
         j.charge = actual;
         j.t_started = Clock::now();
+        if (j.rank && !j.keep_generator) {
+            // fed during the job; a failure here leaves the one-pass stage at the end
+            const uint64_t b = j.desc.d.block, ut = b * (b + 1) / 2;
+            j.seq.resize((1 + 4 * ((j.max_len - 1 + 3) / 4)) * ut);
+            if (pgw_rank_create(j.desc.d.prime, j.desc.d.block, j.rows, j.cols, rank_threshold, rank_buffer,
+                                rank_threads, &j.rk) != PGW_OK)
+                j.rk = nullptr;
+        }
         peak_bytes = std::max(peak_bytes, in_use);
         running.push_back(j.ticket);
         peak_active = std::max<uint64_t>(peak_active, running.size());
@@ -824,8 +878,15 @@ struct pgw_server {
     void collect(uint64_t ticket, const std::string& err) {
         ServerJob* j;
         {
-            std::lock_guard<std::mutex> lk(mtx);
+            // a queued feed is dropped, a running one finished first: the
+            // result below releases the job's device sequence
+            std::unique_lock<std::mutex> lk(mtx);
             j = jobs.at(ticket).get();
+            if (j->feed == 1) {
+                feeds.erase(std::find(feeds.begin(), feeds.end(), ticket));
+                j->feed = 0;
+            }
+            cv_feed.wait(lk, [&] { return j->feed == 0; });
         }
         int rc = PGW_OK;
         std::string msg = err;
@@ -848,6 +909,10 @@ struct pgw_server {
             cv_io.notify_all();
         }
         if (rc == PGW_OK && j->rank) rank_job(*j);
+        if (j->rk) {
+            pgw_rank_destroy(j->rk);
+            j->rk = nullptr;
+        }
         std::lock_guard<std::mutex> lk(mtx);
         finish(*j, rc == PGW_OK ? DONE : FAILED, msg);
         cv_io.notify_all();
@@ -857,6 +922,16 @@ struct pgw_server {
     // 46-57 on a completed file), on this worker thread
     void rank_job(ServerJob& j) {
         const Clock::time_point t = Clock::now();
+        if (j.rk) {
+            // the terms after those fed while the job ran
+            int rc = j.fed < j.L ? pgw_rank_step(j.rk, j.seq.data(), j.L, &j.rank_info)
+                                 : pgw_rank_get_info(j.rk, &j.rank_info);
+            if (rc != PGW_OK) j.rank_error = pgw_last_error();
+            j.rank_rc = rc;
+            j.ranked = true;
+            j.rank_s += std::chrono::duration<double>(Clock::now() - t).count();
+            return;
+        }
         pgw_rank* r = nullptr;
         int rc = pgw_rank_create(j.desc.d.prime, j.desc.d.block, j.rows, j.cols, rank_threshold,
                                  rank_buffer, rank_threads, &r);
@@ -873,6 +948,32 @@ struct pgw_server {
         pgw_rank_destroy(r);
         j.ranked = true;
         j.rank_s = std::chrono::duration<double>(Clock::now() - t).count();
+    }
+
+    // One incremental rank step on the terms the job finished since the last
+    // feed (RankComputation::step, third_party/bcw/src/rank_pipeline.rs:
+    // 153-222, which the reference runs once on a completed file).  A
+    // failure drops the streamed state: the job's rank stage then runs in
+    // one pass after it leaves the GPU.
+    void feed_rank(ServerJob& j) {
+        const Clock::time_point t = Clock::now();
+        uint64_t done = 0;
+        int rc = pgw_batch_terms(batch, j.batch_job, j.fed, j.seq.data(), &done);
+        if (rc == PGW_OK && done > j.fed) {
+            pgw_rank_info info{};
+            rc = pgw_rank_step(j.rk, j.seq.data(), done, &info);
+        }
+        const double secs = std::chrono::duration<double>(Clock::now() - t).count();
+        std::lock_guard<std::mutex> lk(mtx);
+        j.rank_s += secs;
+        if (rc == PGW_OK) {
+            j.fed = std::max(j.fed, done);
+        } else {
+            pgw_rank_destroy(j.rk);
+            j.rk = nullptr;
+        }
+        j.feed = 0;
+        cv_feed.notify_all();
     }
 
     // the GPU thread: batch steps back to back while jobs are running
@@ -899,6 +1000,17 @@ struct pgw_server {
                     still.push_back(t);
             }
             running.swap(still);
+            // running jobs whose finished terms have grown enough feed the rank stage
+            for (uint64_t t : running) {
+                ServerJob& j = *jobs.at(t);
+                uint64_t done = 0;
+                if (!j.rk || j.feed != 0 || pgw_batch_terms(batch, j.batch_job, 0, nullptr, &done) != PGW_OK)
+                    continue;
+                if (done > j.fed && done - j.fed >= std::max(kFeedMin, j.fed)) {
+                    j.feed = 1;
+                    feeds.push_back(t);
+                }
+            }
             cv_io.notify_all();
         }
     }
@@ -1086,8 +1198,10 @@ int pgw_server_destroy(pgw_server* s) {
         }
         s->cv_step.notify_all();
         if (s->stepper.joinable()) s->stepper.join();
-        for (auto& kv : s->jobs)
+        for (auto& kv : s->jobs) {
             if (kv.second->op) pgw_operator_destroy(kv.second->op);
+            if (kv.second->rk) pgw_rank_destroy(kv.second->rk);
+        }
         pgw_batch_destroy(s->batch);
         delete s;
     });

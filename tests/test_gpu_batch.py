@@ -291,6 +291,66 @@ def test_server_rank_stage_matches_standalone_consumer(ctx, g12, g12_reference):
     srv.close()
 
 
+def test_batch_terms_of_running_jobs_are_final(ctx):
+    # terms read while a job still runs (pgw_batch_terms) equal its final sequence
+    jobs = small_mix(count=2)
+    batch = pg.Batch(ctx, use_graph=True, grouped=True)
+    ids, ops, live = [], [], []
+    for j in jobs:
+        op, st = make_op(ctx, j, pg.METHOD_MFMA)
+        L = pg.sequence_length(41)
+        ids.append(batch.add(op, st[4], 41))
+        ops.append(op)
+        live.append(np.zeros((L, j.block * (j.block + 1) // 2), np.uint32))
+    assert all(batch.terms(i) == 1 for i in ids)  # V^T V before any replay
+    fed = [0, 0]
+    while batch.step(2):
+        for q, i in enumerate(ids):
+            done = batch.terms(i, fed[q], live[q])
+            assert done >= fed[q] and (done - 1) % 4 == 0
+            fed[q] = done
+    for q, i in enumerate(ids):
+        fed[q] = batch.terms(i, fed[q], live[q])
+        seq, _ = batch.result(i)
+        assert fed[q] == seq.shape[0] and (live[q] == seq).all()
+        ops[q].close()
+    batch.close()
+
+
+@pytest.mark.parametrize("threads", [1, 2])
+def test_server_rank_stage_fed_during_jobs_matches_one_pass(ctx, threads):
+    # jobs long enough for the rank stage to be fed while they run (steps at
+    # 512, 1536 and 3072 terms, then the rest): the rank bound, gap and status
+    # equal one pass of the standalone consumer over the final sequence
+    from math import comb
+    srv = pg.Server(ctx, 8 << 30, max_active=4, replays_per_step=4)
+    srv.enable_rank(threads=threads, keep_generator=False)
+    tickets = []
+    for (n, k, M, ps, bs, seed) in [(10, 4, 40, 65521, 4, 0), (9, 3, 30, 2147483647, 8, 1)]:
+        Mp = M + (M + 9) // 10
+        spec = pg.SyntheticSpec(n, k, M, Mp, ps, seed, seed + 1)
+        T, E = spec.tables()
+        N = comb(n, k) * M
+        dr, dc, v = pg.reference_random_state(1, seed, ps, N, N, bs)
+        pairs = pg.rowmix_plan(pg.rowmix_realized_seed(10001, "synthetic", 0, N, ps), N, 2, ps)
+        desc = pg.Operator.synthetic_desc(spec, bs, pg.METHOD_AUTO, T, E)
+        L = 2 * N // bs + 8
+        _log(f"server fed rank stage: N={N} b={bs} p={ps} terms={L}")
+        tickets.append((srv.submit(desc, dr, dc, v, L, 2, pairs, keep=(T, E)),
+                        comb(n, k - 1) * Mp, N, ps, bs))
+    for t, want, N, prime, blk in tickets:
+        assert srv.wait(t, 600.0) == pg.Server.DONE
+        st, gen, secs = srv.rank(t)
+        seq, _, _ = srv.result(t)
+        assert gen is None and secs > 0
+        lone = pg.RankComputation(prime, blk, N, N)
+        lst = lone.step(seq)
+        assert (st.rank_lower_bound, st.delta_spread, st.status) == \
+            (lst.rank_lower_bound, lst.delta_spread, lst.status), (t, st, lst)
+        assert st.rank_lower_bound == want and st.status == pg.RANK_SUCCESS, (t, st)
+    srv.close()
+
+
 def test_server_refuses_job_larger_than_budget(ctx):
     j = small_mix(count=1)[0]
     dr, dc, v, pairs = wl.sector_state(j, 3)
