@@ -272,7 +272,8 @@ def bench_serve(args):
     the others stop at --max-len) and, unless --no-rank, the host rank stage
     on its sequence.  Reports
     submit-to-finish latency p50/p99 and throughput, against the same stream
-    through a server that runs one job at a time."""
+    through a server that runs one job at a time (and, with --plain-baseline,
+    one job at a time on the plain u64 path)."""
     import prym_green_ancillary_amd as pg
     from prym_green_ancillary_amd import workloads as wl
 
@@ -288,16 +289,20 @@ def bench_serve(args):
         full = j.full_length()
         L = full if full <= args.full_max else min(full, args.max_len)
         prepared.append((j, desc, (T, E), dr, dc, v, pairs, L))
+    plain_descs = [pg.Operator.synthetic_desc(j.spec, j.block, pg.METHOD_PLAIN, *keep)
+                   for (j, _, keep, *_rest) in prepared] if args.plain_baseline else None
     t_setup = time.perf_counter() - t_setup
     budget = int(args.hbm_fraction * total)
 
-    def replay(max_active):
+    def replay(max_active, descs=None):
         srv = pg.Server(ctx, budget, max_active=max_active, replays_per_step=1)
         if not args.no_rank:
             srv.enable_rank(threads=args.rank_threads)
         t0 = time.perf_counter()
         tickets = []
-        for j, desc, keep, dr, dc, v, pairs, L in prepared:
+        for q, (j, desc, keep, dr, dc, v, pairs, L) in enumerate(prepared):
+            if descs is not None:
+                desc = descs[q]
             delay = j.arrival - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
@@ -323,10 +328,15 @@ def bench_serve(args):
 
     mixed, wall, lat, stats = replay(args.max_active)
     alone, wall1, lat1, _ = replay(1)
-    for i, (a, b1) in enumerate(zip(mixed, alone)):
-        if not ((a[0] == b1[0]).all() and (a[1] == b1[1]).all()) or a[3] != b1[3]:
-            print(f"job {i}: results differ between the mix and one-at-a-time", file=sys.stderr)
-            sys.exit(1)
+    others = [alone]
+    if plain_descs is not None:
+        plain, wallp, latp, _ = replay(1, plain_descs)
+        others.append(plain)
+    for other in others:
+        for i, (a, b1) in enumerate(zip(mixed, other)):
+            if not ((a[0] == b1[0]).all() and (a[1] == b1[1]).all()) or a[3] != b1[3]:
+                print(f"job {i}: results differ between the mix and a one-at-a-time run", file=sys.stderr)
+                sys.exit(1)
     applies = sum(2 * ((L - 1 + 3) // 4) for (_, _, _, _, _, _, _, L) in prepared)
     Ns = [j.cols for j in jobs]
     is_full = [L >= j.full_length() for (j, _, _, _, _, _, _, L) in prepared]
@@ -362,6 +372,11 @@ def bench_serve(args):
                           "latency_p50_s": round(float(np.percentile(lat1, 50)), 3),
                           "latency_p99_s": round(float(np.percentile(lat1, 99)), 3),
                           "jobs_per_s": round(len(jobs) / wall1, 3)},
+        "plain_one_at_a_time": None if plain_descs is None else {
+            "mean_phase_s": phases(plain),
+            "latency_p50_s": round(float(np.percentile(latp, 50)), 3),
+            "latency_p99_s": round(float(np.percentile(latp, 99)), 3),
+            "jobs_per_s": round(len(jobs) / wallp, 3)},
         "peak_active": stats["peak_active"], "peak_bytes": stats["peak_bytes"],
         "bit_exact_mix_vs_alone": True,
         "rank_stage": None if args.no_rank else {
@@ -475,7 +490,7 @@ def main():
                     help="large: the N=53.8M sector (default); sector: the mid-size sector; "
                          "batch: 64 uneven sectors; serve: Poisson job stream; paper: the "
                          "reference's g20/g22 instances")
-    ap.add_argument("--genus", type=int, default=22, choices=[20, 22, 24], help="paper: instance")
+    ap.add_argument("--genus", type=int, default=22, choices=[20, 22, 24, 26], help="paper: instance")
     ap.add_argument("--paper-sectors", default="", help="paper: comma-separated sectors (default all)")
     ap.add_argument("--eager-rounds", type=int, default=0,
                     help="paper: time this many eager apply+Gram rounds on the first sector instead")
@@ -490,6 +505,8 @@ def main():
     ap.add_argument("--hbm-fraction", type=float, default=0.5, help="serve: HBM budget share")
     ap.add_argument("--no-rank", action="store_true", help="serve: skip the host rank stage")
     ap.add_argument("--rank-threads", type=int, default=8, help="serve: rank-stage threads per job")
+    ap.add_argument("--plain-baseline", action="store_true",
+                    help="serve: also replay the stream one job at a time on the plain u64 path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); the sector path "
