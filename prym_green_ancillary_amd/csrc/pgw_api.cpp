@@ -127,9 +127,10 @@ void timed_limb_split(pgw_op* op, const uint32_t* in, const uint32_t* scale, int
     t.done();
 }
 
-void timed_sell(pgw_op* op, const SellDevice& M, const uint32_t* in, uint32_t* out, hipStream_t st) {
-    StageTimer t(op, 4, sell_bytes(M, op->b), st);
-    launch_sell_spmm(M, in, out, op->b, op->m, st);
+void timed_sell(pgw_op* op, const SellDevice& M, const uint32_t* in, uint32_t* out, hipStream_t st,
+                const uint32_t* out_row = nullptr) {
+    StageTimer t(op, 4, sell_bytes(M, op->b) + (out_row ? M.rows * 4.0 : 0.0), st);
+    launch_sell_spmm(M, in, out, op->b, op->m, st, out_row);
     t.done();
 }
 
@@ -194,10 +195,13 @@ void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
 
 void pgw::apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st) {
     if (op->kind != PGW_KIND_SYNTHETIC) {
-        // S = M1^T M2^T M3 M2 M1: Dc, the elimination and P^T Dr P are folded in
+        // S = M1^T M2^T M3 M2 M1: Dc, the elimination and P^T Dr P are folded in.
+        // With the row-mixing order, M2 stores t by position, M3 (kept in that
+        // order) gathers from nearby positions and stores its rows back in
+        // natural order, where M2^T gathers them.
         timed_sell(op, op->m1, in, op->d_y, st);
-        timed_sell(op, op->m2, op->d_y, op->d_target, st);
-        timed_sell(op, op->m3, op->d_target, op->d_t2, st);
+        timed_sell(op, op->m2, op->d_y, op->d_target, st, op->d_t_pos);
+        timed_sell(op, op->m3, op->d_target, op->d_t2, st, op->d_t_row);
         timed_sell(op, op->m2t, op->d_t2, op->d_y, st);
         timed_sell(op, op->m1t, op->d_y, out, st);
         return;
@@ -604,7 +608,21 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             op->m2 = upload_sell(A, maps.m2, op->p);
             op->m2t = upload_sell(A, csr_transpose(maps.m2), op->p);
             maps.m2 = CsrHost();
-            op->m3 = upload_sell(A, maps.m3, op->p);
+            // M3 = P^T Dr P gathers each row's partners through the pairing,
+            // at random in natural order (0.93 cache-line requests per entry
+            // at g24) but from a few positions away in the row-mixing order
+            // (0.26); PGW_CYCLIC_ORDER=0 keeps the natural order
+            const char* knob = std::getenv("PGW_CYCLIC_ORDER");
+            if (op->rounds > 0 && !(knob && knob[0] == '0')) {
+                const std::vector<uint32_t> order = mix_order(op->rows, op->rounds, h_pairs);
+                std::vector<uint32_t> pos(op->rows);
+                for (uint64_t i = 0; i < op->rows; ++i) pos[order[i]] = (uint32_t)i;
+                op->m3 = upload_sell(A, permute_square(maps.m3, order), op->p);
+                op->d_t_row = A.upload(order);
+                op->d_t_pos = A.upload(pos);
+            } else {
+                op->m3 = upload_sell(A, maps.m3, op->p);
+            }
             op->d_y = A.alloc<uint32_t>(op->y_rows * b);
             op->d_t2 = A.alloc<uint32_t>(op->rows * b);
         }

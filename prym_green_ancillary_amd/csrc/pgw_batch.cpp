@@ -377,8 +377,6 @@ struct pgw_batch {
     }
 };
 
-extern "C" {
-
 namespace {
 
 // The operator's shape as far as the sweeps' schedules go: subsets, panel
@@ -406,6 +404,8 @@ void tune_for_shape(pgw_batch* b, pgw_op* op) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int pgw_batch_create(pgw_ctx* ctx, uint32_t flags, pgw_batch** out) {
     pgw_batch* b = nullptr;
@@ -742,8 +742,8 @@ uint64_t estimate_job_bytes(const pgw_operator_desc& d, uint64_t rows, uint64_t 
     } else {
         // y', the mixed target and the five sparse maps of the chain (sliced
         // ELL, 8 bytes per entry; rows of the paper instances hold < 100
-        // entries per map, padding included)
-        bytes += 3 * (rows + cols + 2) * b * 4 + 5 * 100 * 8 * (rows + cols + 64);
+        // entries per map, padding included), the row-mixing order of t
+        bytes += 3 * (rows + cols + 2) * b * 4 + 5 * 100 * 8 * (rows + cols + 64) + 8 * rows;
     }
     return bytes + (16ull << 20);
 }

@@ -279,6 +279,15 @@ struct CyclicMaps {
 void build_cyclic_maps(CyclicMaps& out, const CyclicLayout& L, const pgw_operator_desc& d,
                        uint64_t cols, const uint32_t* dc, const uint32_t* dr, uint32_t rounds,
                        const pgw_rowmix_pair* pairs);
+// An order of the `rows` target rows in which the row-mixing rounds are
+// local: every pair of every round links two rows, and the order walks the
+// linked components (with two rounds: paths and cycles, so that each row's
+// partners, and theirs, sit a few positions away).  order[i] = the row at
+// position i.
+std::vector<uint32_t> mix_order(uint64_t rows, uint32_t rounds, const pgw_rowmix_pair* pairs);
+// A with rows and columns renumbered: row i of the result is row order[i] of
+// A, column c becomes the position of c in `order` (square A).
+CsrHost permute_square(const CsrHost& A, const std::vector<uint32_t>& order);
 // out = S in on a cols x w host block, matrix-free from the layout (pgw_cpu.cpp)
 void cyclic_apply_host(const CyclicLayout& L, const pgw_operator_desc& d, uint64_t cols, const uint32_t* dr,
                        const uint32_t* dc, uint32_t rounds, const pgw_rowmix_pair* pairs, const uint32_t* in,

@@ -70,9 +70,11 @@ struct SellDevice {
     const uint32_t* ent = nullptr;
     uint32_t pack_bits = 0;
 };
-// out (M.rows x b) = M in (M.cols x b) mod p; out must not overlap in
+// out (M.rows x b) = M in (M.cols x b) mod p; out must not overlap in.  With
+// out_row, row r of M lands at row out_row[r] of out (a permutation of the
+// rows; a scattered 16-byte store per row instead of a coalesced one).
 void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
-                      const ModP& m, hipStream_t st);
+                      const ModP& m, hipStream_t st, const uint32_t* out_row = nullptr);
 
 // elementwise -----------------------------------------------------------------
 void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uint64_t rows,

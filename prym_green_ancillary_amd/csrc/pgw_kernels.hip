@@ -183,16 +183,23 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 // the lanes of a wavefront that work on consecutive rows read consecutive
 // entries.  A row is handled by B / CW column groups (a 16-byte load gathers
 // four columns) times LANES lanes that split its entries (lane l takes entries
-// l, l + LANES, ...) and add their sums with cross-lane shuffles: short rows of
-// small sectors get enough threads to hide the entry -> gather latency chain.
+// l, l + LANES, ...) and add their sums with cross-lane shuffles.
+// Every entry is a dependent pair of loads (the entry, then the input row it
+// names), so a lane takes its entries U at a time: U entry loads, then U
+// gathers, all in flight together before the first product.  The loads carry
+// no branch (a branch around a load makes the compiler wait for it at the
+// join): an entry past the row's slice width re-reads the last entry of the
+// row and is then zeroed, so its gather reads input row 0, which exists
+// whenever the slice holds any entry.
 // SMALLP (p < 2^16): products < 2^32, summed in u64; otherwise products
 // < 2^62 summed as 64-bit low/high word sums.  Exact for rows of any length
 // below 2^32.
-template <int B, int CW, int LANES, bool SMALLP, bool PACKED>
+template <int B, int CW, int LANES, int U, bool SMALLP, bool PACKED>
 __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ slice,
                                                    const uint32_t* __restrict__ ent, uint64_t rows,
                                                    const uint32_t* __restrict__ in,
-                                                   uint32_t* __restrict__ out, ModP m, uint32_t pack_bits) {
+                                                   uint32_t* __restrict__ out, ModP m, uint32_t pack_bits,
+                                                   const uint32_t* __restrict__ out_row) {
     constexpr int G = B / CW;
     constexpr int T = G * LANES;  // threads per row; LANES > 1 only when T divides 64
     const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -207,33 +214,44 @@ __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ 
         const uint64_t b0 = slice[sl], width = (slice[sl + 1] - b0) / kSellHeight;
         const uint64_t e0 = b0 + row % kSellHeight;
         const uint32_t col_mask = PACKED ? (1u << pack_bits) - 1u : 0u;
-        for (uint64_t k = lane; k < width; k += LANES) {
-            uint2 c;
-            if constexpr (PACKED) {
-                const uint32_t w = ent[e0 + k * kSellHeight];
-                c = make_uint2(w & col_mask, w >> pack_bits);
-            } else {
-                c = reinterpret_cast<const uint2*>(ent)[e0 + k * kSellHeight];
-            }
-            const uint32_t* xr = in + (uint64_t)c.x * B + g * CW;
-            uint32_t xv[CW];
-            if constexpr (CW == 4) {
-                const uint4 v = *reinterpret_cast<const uint4*>(xr);
-                xv[0] = v.x; xv[1] = v.y; xv[2] = v.z; xv[3] = v.w;
-            } else {
+        for (uint64_t k0 = lane; k0 < width; k0 += (uint64_t)LANES * U) {
+            uint2 c[U];
 #pragma unroll
-                for (int q = 0; q < CW; ++q) xv[q] = xr[q];
-            }
-#pragma unroll
-            for (int q = 0; q < CW; ++q) {
-                const uint64_t prod = (uint64_t)c.y * xv[q];
-                if (SMALLP) {
-                    lo[q] += prod;
+            for (int u = 0; u < U; ++u) {
+                const uint64_t k = k0 + (uint64_t)u * LANES;
+                const uint64_t e = e0 + (k < width ? k : width - 1) * kSellHeight;
+                if constexpr (PACKED) {
+                    const uint32_t w = ent[e];
+                    c[u] = make_uint2(w & col_mask, w >> pack_bits);
                 } else {
-                    lo[q] += (uint32_t)prod;
-                    hi[q] += prod >> 32;
+                    c[u] = reinterpret_cast<const uint2*>(ent)[e];
+                }
+                if (k >= width) c[u] = make_uint2(0u, 0u);
+            }
+            uint32_t xv[U][CW];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t* xr = in + (uint64_t)c[u].x * B + g * CW;
+                if constexpr (CW == 4) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(xr);
+                    xv[u][0] = v.x; xv[u][1] = v.y; xv[u][2] = v.z; xv[u][3] = v.w;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < CW; ++q) xv[u][q] = xr[q];
                 }
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int q = 0; q < CW; ++q) {
+                    const uint64_t prod = (uint64_t)c[u].y * xv[u][q];
+                    if (SMALLP) {
+                        lo[q] += prod;
+                    } else {
+                        lo[q] += (uint32_t)prod;
+                        hi[q] += prod >> 32;
+                    }
+                }
         }
     }
     if constexpr (LANES > 1) {
@@ -247,20 +265,24 @@ __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ 
             }
     }
     if (valid && lane == 0) {
+        const uint64_t orow = out_row ? out_row[row] : row;
 #pragma unroll
         for (int q = 0; q < CW; ++q)
-            out[row * B + g * CW + q] = SMALLP ? mod_u64(lo[q], m) : mod_split(hi[q], lo[q], m);
+            out[orow * B + g * CW + q] = SMALLP ? mod_u64(lo[q], m) : mod_split(hi[q], lo[q], m);
     }
 }
 
+// entries a lane keeps in flight (k_sell_spmm's U)
+constexpr int kSellBatch = 8;
+
 template <int B, int CW, int LANES>
 static void sell_launch(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
-                        hipStream_t st) {
+                        hipStream_t st, const uint32_t* out_row) {
     const uint64_t threads = M.rows * (B / CW) * LANES;
     const dim3 grid(grid_for(threads)), blk(256);
-#define PGW_SELL(SP, PK)                                                                                   \
-    hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, SP, PK>), grid, blk, 0, st, M.slice, M.ent, M.rows, in, out, \
-                       m, M.pack_bits)
+#define PGW_SELL(SP, PK)                                                                                  \
+    hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, kSellBatch, SP, PK>), grid, blk, 0, st, M.slice, M.ent,     \
+                       M.rows, in, out, m, M.pack_bits, out_row)
     if (m.p < 65536u) {
         if (M.pack_bits) PGW_SELL(true, true); else PGW_SELL(true, false);
     } else {
@@ -269,50 +291,53 @@ static void sell_launch(const SellDevice& M, const uint32_t* in, uint32_t* out, 
 #undef PGW_SELL
 }
 
-// lanes per row: enough threads to fill the chip (about 2^20) when rows are
-// few, never more than the map's mean padded row width; 1 when B / CW is not
-// a power of two (a row's threads must stay inside one wavefront)
+// lanes per row: with kSellBatch entries in flight per lane, one lane per row
+// keeps the memory system busy from about 2^16 threads on (g22's 84 k rows:
+// 37.4 us per apply with one lane, 37.7 with two, 51.4 with four; g24's 323 k
+// rows: 69 / 75 / 83 us); fewer rows get more lanes, never more than the
+// map's mean padded row width.  1 when B / CW is not a power of two (a row's
+// threads must stay inside one wavefront)
 template <int B, int CW>
 static void sell_b(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
-                   hipStream_t st) {
+                   hipStream_t st, const uint32_t* out_row) {
     constexpr int G = B / CW;
     int lanes = 1;
     if ((G & (G - 1)) == 0) {
         const double width = M.rows ? (double)M.padded / M.rows : 0.0;
-        while (lanes < 8 && (double)M.rows * G * lanes < (double)(1 << 20) && 2.0 * lanes <= width)
+        while (lanes < 8 && (double)M.rows * G * lanes < (double)(1 << 16) && 2.0 * lanes <= width)
             lanes *= 2;
     }
     if constexpr ((G & (G - 1)) == 0 && G <= 8) {
-        if (lanes == 8) return sell_launch<B, CW, 8>(M, in, out, m, st);
-        if (lanes == 4) return sell_launch<B, CW, 4>(M, in, out, m, st);
-        if (lanes == 2) return sell_launch<B, CW, 2>(M, in, out, m, st);
+        if (lanes == 8) return sell_launch<B, CW, 8>(M, in, out, m, st, out_row);
+        if (lanes == 4) return sell_launch<B, CW, 4>(M, in, out, m, st, out_row);
+        if (lanes == 2) return sell_launch<B, CW, 2>(M, in, out, m, st, out_row);
     }
-    sell_launch<B, CW, 1>(M, in, out, m, st);
+    sell_launch<B, CW, 1>(M, in, out, m, st, out_row);
 }
 
 void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
-                      const ModP& m, hipStream_t st) {
+                      const ModP& m, hipStream_t st, const uint32_t* out_row) {
     if (M.rows == 0) return;
     require(M.rows < (1ULL << 40), "sparse map too large");
     // 16-byte gathers need 16-byte aligned rows
     const bool vec = b % 4 == 0 && (reinterpret_cast<uintptr_t>(in) % 16) == 0;
     switch (b) {
-        case 1: sell_b<1, 1>(M, in, out, m, st); break;
-        case 2: sell_b<2, 2>(M, in, out, m, st); break;
-        case 3: sell_b<3, 3>(M, in, out, m, st); break;
-        case 4: if (vec) sell_b<4, 4>(M, in, out, m, st); else sell_b<4, 1>(M, in, out, m, st); break;
-        case 5: sell_b<5, 1>(M, in, out, m, st); break;
-        case 6: sell_b<6, 2>(M, in, out, m, st); break;
-        case 7: sell_b<7, 1>(M, in, out, m, st); break;
-        case 8: if (vec) sell_b<8, 4>(M, in, out, m, st); else sell_b<8, 1>(M, in, out, m, st); break;
-        case 9: sell_b<9, 3>(M, in, out, m, st); break;
-        case 10: sell_b<10, 2>(M, in, out, m, st); break;
-        case 11: sell_b<11, 1>(M, in, out, m, st); break;
-        case 12: if (vec) sell_b<12, 4>(M, in, out, m, st); else sell_b<12, 1>(M, in, out, m, st); break;
-        case 13: sell_b<13, 1>(M, in, out, m, st); break;
-        case 14: sell_b<14, 2>(M, in, out, m, st); break;
-        case 15: sell_b<15, 3>(M, in, out, m, st); break;
-        case 16: if (vec) sell_b<16, 4>(M, in, out, m, st); else sell_b<16, 1>(M, in, out, m, st); break;
+        case 1: sell_b<1, 1>(M, in, out, m, st, out_row); break;
+        case 2: sell_b<2, 2>(M, in, out, m, st, out_row); break;
+        case 3: sell_b<3, 3>(M, in, out, m, st, out_row); break;
+        case 4: if (vec) sell_b<4, 4>(M, in, out, m, st, out_row); else sell_b<4, 1>(M, in, out, m, st, out_row); break;
+        case 5: sell_b<5, 1>(M, in, out, m, st, out_row); break;
+        case 6: sell_b<6, 2>(M, in, out, m, st, out_row); break;
+        case 7: sell_b<7, 1>(M, in, out, m, st, out_row); break;
+        case 8: if (vec) sell_b<8, 4>(M, in, out, m, st, out_row); else sell_b<8, 1>(M, in, out, m, st, out_row); break;
+        case 9: sell_b<9, 3>(M, in, out, m, st, out_row); break;
+        case 10: sell_b<10, 2>(M, in, out, m, st, out_row); break;
+        case 11: sell_b<11, 1>(M, in, out, m, st, out_row); break;
+        case 12: if (vec) sell_b<12, 4>(M, in, out, m, st, out_row); else sell_b<12, 1>(M, in, out, m, st, out_row); break;
+        case 13: sell_b<13, 1>(M, in, out, m, st, out_row); break;
+        case 14: sell_b<14, 2>(M, in, out, m, st, out_row); break;
+        case 15: sell_b<15, 3>(M, in, out, m, st, out_row); break;
+        case 16: if (vec) sell_b<16, 4>(M, in, out, m, st, out_row); else sell_b<16, 1>(M, in, out, m, st, out_row); break;
         default: throw Error(PGW_ERR_UNSUPPORTED, "block width must be in 1..16");
     }
     PGW_CHECK_LAUNCH("sparse map");

@@ -120,6 +120,11 @@ struct pgw_op {
     // (explicit sparse maps, pgw_sparse.cpp); d_y holds y' (M1's rows), d_t2
     // the mixed target block
     SellDevice m1, m2, m3, m2t, m1t;
+    // t in the row-mixing order (mix_order): d_t_pos[row] = its position,
+    // d_t_row[position] = its row; m3 is then stored in that order.  Null:
+    // t and m3 in natural row order.
+    const uint32_t* d_t_pos = nullptr;
+    const uint32_t* d_t_row = nullptr;
     uint64_t y_rows = 0;
     uint32_t* d_y = nullptr;
     uint32_t* d_t2 = nullptr;

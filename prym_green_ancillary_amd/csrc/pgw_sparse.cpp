@@ -274,4 +274,70 @@ void build_cyclic_maps(CyclicMaps& out, const CyclicLayout& L, const pgw_operato
     out.m3 = flatten(m3, rows, p);
 }
 
+std::vector<uint32_t> mix_order(uint64_t rows, uint32_t rounds, const pgw_rowmix_pair* pairs) {
+    const uint64_t pc = rows / 2;
+    // partner of each row in each round (~0: unpaired in that round)
+    std::vector<uint32_t> partner((uint64_t)rounds * rows, ~0u);
+    for (uint32_t r = 0; r < rounds; ++r)
+        for (uint64_t q = 0; q < pc; ++q) {
+            const pgw_rowmix_pair& pr = pairs[(uint64_t)r * pc + q];
+            partner[(uint64_t)r * rows + pr.first] = pr.second;
+            partner[(uint64_t)r * rows + pr.second] = pr.first;
+        }
+    std::vector<uint32_t> order;
+    order.reserve(rows);
+    std::vector<char> seen(rows, 0);
+    // from `start`, step to an unseen partner (first round first) until none
+    // is left; with at most two rounds that traces a whole path or cycle
+    auto walk = [&](uint64_t start) {
+        uint64_t cur = start;
+        for (;;) {
+            seen[cur] = 1;
+            order.push_back((uint32_t)cur);
+            uint64_t next = ~0ULL;
+            for (uint32_t r = 0; r < rounds && next == ~0ULL; ++r) {
+                const uint32_t c = partner[(uint64_t)r * rows + cur];
+                if (c != ~0u && !seen[c]) next = c;
+            }
+            if (next == ~0ULL) return;
+            cur = next;
+        }
+    };
+    // path ends (a row unpaired in some round) first, so paths are walked end to end
+    for (uint64_t v = 0; v < rows; ++v) {
+        if (seen[v]) continue;
+        bool end = false;
+        for (uint32_t r = 0; r < rounds; ++r) end |= partner[(uint64_t)r * rows + v] == ~0u;
+        if (end) walk(v);
+    }
+    for (uint64_t v = 0; v < rows; ++v)
+        if (!seen[v]) walk(v);
+    return order;
+}
+
+CsrHost permute_square(const CsrHost& A, const std::vector<uint32_t>& order) {
+    require(A.rows == A.cols && order.size() == A.rows, "permutation does not match the map");
+    std::vector<uint32_t> pos(A.rows);
+    for (uint64_t i = 0; i < A.rows; ++i) pos[order[i]] = (uint32_t)i;
+    CsrHost P;
+    P.rows = P.cols = A.rows;
+    P.ptr.assign(A.rows + 1, 0);
+    for (uint64_t i = 0; i < A.rows; ++i) P.ptr[i + 1] = P.ptr[i] + (A.ptr[order[i] + 1] - A.ptr[order[i]]);
+    P.col.resize(P.ptr[A.rows]);
+    P.val.resize(P.ptr[A.rows]);
+    parallel_rows(A.rows, [&](uint64_t lo, uint64_t hi) {
+        std::vector<std::pair<uint32_t, uint32_t>> row;
+        for (uint64_t i = lo; i < hi; ++i) {
+            row.clear();
+            for (uint64_t k = A.ptr[order[i]]; k < A.ptr[order[i] + 1]; ++k) row.emplace_back(pos[A.col[k]], A.val[k]);
+            std::sort(row.begin(), row.end());
+            for (size_t t = 0; t < row.size(); ++t) {
+                P.col[P.ptr[i] + t] = row[t].first;
+                P.val[P.ptr[i] + t] = row[t].second;
+            }
+        }
+    });
+    return P;
+}
+
 }  // namespace pgw

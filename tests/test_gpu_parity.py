@@ -72,13 +72,18 @@ def gpu_apply(torch, op, x):
 
 # ------------------------------------------------------------ cyclic (g12) --
 
-@pytest.mark.parametrize("b,packed", [(4, True), (3, True), (4, False)])
-def test_g12_apply_square_all_sectors(torch_cuda, ctx, g12, b, packed, monkeypatch):
+@pytest.mark.parametrize("b,packed,mix_order", [(4, True, True), (3, True, True), (4, False, True),
+                                                (4, True, False), (5, False, False)])
+def test_g12_apply_square_all_sectors(torch_cuda, ctx, g12, b, packed, mix_order, monkeypatch):
     # packed: sparse-map entries as one u32 (column | coefficient << column
     # bits), which g12's sizes allow; PGW_SELL_PACK=0 keeps (column,
-    # coefficient) pairs, the form of instances too large to pack
+    # coefficient) pairs, the form of instances too large to pack.
+    # mix_order: t and M3 = P^T Dr P in the row-mixing order (the default);
+    # PGW_CYCLIC_ORDER=0 keeps the natural row order
     if not packed:
         monkeypatch.setenv("PGW_SELL_PACK", "0")
+    if not mix_order:
+        monkeypatch.setenv("PGW_CYCLIC_ORDER", "0")
     p = g12["modulus"]
     for sector in range(g12["cyclic"]["order"]):
         _log(f"g12 sector {sector} b={b}")
@@ -112,7 +117,10 @@ def test_g12_certificate_on_gpu(torch_cuda, ctx, g12, g12_reference):
     assert host(d_out).any()
 
 
-def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12):
+@pytest.mark.parametrize("mix_order", [True, False])
+def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12, mix_order, monkeypatch):
+    if not mix_order:
+        monkeypatch.setenv("PGW_CYCLIC_ORDER", "0")
     p, sector, b = g12["modulus"], 1, 4
     o = oc.cyclic_op(g12, sector)
     dr, dc, v = pg.reference_random_state(1, sector, p, o.rows, o.cols, b)

@@ -4,10 +4,10 @@ CPU: the host layout of every eliminated sector reproduces the sector shapes
 that data/cases.json records (and the elimination checks mu_w R = I,
 mu_w B = 0 pass while it is built).
 
-GPU: the generator CLI runs every sector of genus 20 and genus 22 with the
-paper recipe (b = 4, seed 1, two row-mixing rounds from seed 10001, length
-2N/4 + 50) and prym-rank reads each WDM file; every rank equals
-cases.json's expected_base_ranks.  The median sequence-loop time per genus
+GPU: the generator CLI runs every sector of genus 20 and genus 22, and
+sectors 0 and 1 of genus 24, with the paper recipe (b = 4, seed 1, two
+row-mixing rounds from seed 10001, length 2N/4 + 50) and prym-rank reads each
+WDM file; every rank equals cases.json's expected_base_ranks.  The median sequence-loop time per genus
 is printed beside the published one (other hardware: context only).
 """
 import json
@@ -102,3 +102,12 @@ def test_g22_every_sector_rank_matches_cases(tmp_path):
     case = _case(22)
     ranks, _ = _run_genus(tmp_path, 22, range(case["order"]))
     assert [ranks[s] for s in range(case["order"])] == case["expected_base_ranks"]
+
+
+@pytest.mark.gpu
+def test_g24_sectors_0_1_rank_matches_cases(tmp_path):
+    # both row counts of g24 (323,302 and 323,344 rows; 147,015 sequence
+    # terms each), the largest instance ranked end to end here
+    case = _case(24)
+    ranks, _ = _run_genus(tmp_path, 24, [0, 1])
+    assert [ranks[0], ranks[1]] == case["expected_base_ranks"][:2]
