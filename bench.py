@@ -418,6 +418,7 @@ def bench_paper(args):
     ctx = pg.Context(0)
     dev_name, _ = ctx.device_info()
     loops, per_sector = [], []
+    roofline = None
     for s in sectors:
         d, keep = pg.Operator.cyclic_desc(inst, s, b)
         rows, cols = pg.Operator.dims(d)
@@ -446,6 +447,8 @@ def bench_paper(args):
         t0 = time.perf_counter()
         seq, _ = op.sequence(v, max_len)
         t = time.perf_counter() - t0
+        if roofline is None:
+            roofline = sparse_roofline(op, v, s)
         op.close()
         applies = (seq.shape[0] - 1) // 2
         loops.append(t)
@@ -471,9 +474,27 @@ def bench_paper(args):
         "median_block_applies_per_s": round(float(np.median([x["block_applies_per_s"] for x in per_sector])), 1),
         "per_sector": per_sector,
         "device": dev_name,
-        "roofline": None, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def sparse_roofline(op, v, sector, iters=20):
+    """HBM roofline of the cyclic apply's sparse maps (k_sell_spmm, the
+    dominant kernel): pgw_bench_stages times every launch of `iters` applies
+    with HIP events on the operator's stream; the bytes are each map's
+    algorithmic ones (entries once, input and output blocks once)."""
+    import torch
+    a = torch.from_numpy(v.view(np.int32).copy()).cuda()
+    bb = torch.empty_like(a)
+    stages = op.bench_stages(a.data_ptr(), bb.data_ptr(), iters)
+    ms, by, nlaunch = stages["sparse_maps"]
+    gbs = by / (ms * 1e-3) / 1e9
+    peak = 8000.0  # GB/s, MI355X HBM3E
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": peak, "unit": "GB/s",
+            "frac": round(gbs / peak, 4), "traffic": None, "kernel": "k_sell_spmm",
+            "sector": sector, "ms_per_apply": round(ms, 4), "launches_per_apply": nlaunch,
+            "bytes_per_apply": by}
 
 
 def main():

@@ -193,16 +193,32 @@ void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
 
 }  // namespace
 
+namespace {
+
+// S = M1^T M2^T M3 M2 M1 up to the last map: Dc, the elimination and
+// P^T Dr P are folded in.  With the row-mixing order, M2 stores t by
+// position, M3 (kept in that order) gathers from nearby positions and stores
+// its rows back in natural order, where M2^T gathers them.  Leaves M2^T M3 M2
+// M1 in in op->d_y.
+void cyclic_front(pgw_op* op, const uint32_t* in, hipStream_t st) {
+    timed_sell(op, op->m1, in, op->d_y, st);
+    timed_sell(op, op->m2, op->d_y, op->d_target, st, op->d_t_pos);
+    timed_sell(op, op->m3, op->d_target, op->d_t2, st, op->d_t_row);
+    timed_sell(op, op->m2t, op->d_t2, op->d_y, st);
+}
+
+// PGW_CYCLIC_FUSED_GRAM=0 keeps the projection in its own launches (a test
+// knob, read when a round is enqueued or captured)
+bool fused_gram_allowed() {
+    const char* knob = std::getenv("PGW_CYCLIC_FUSED_GRAM");
+    return !(knob && knob[0] == '0');
+}
+
+}  // namespace
+
 void pgw::apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st) {
     if (op->kind != PGW_KIND_SYNTHETIC) {
-        // S = M1^T M2^T M3 M2 M1: Dc, the elimination and P^T Dr P are folded in.
-        // With the row-mixing order, M2 stores t by position, M3 (kept in that
-        // order) gathers from nearby positions and stores its rows back in
-        // natural order, where M2^T gathers them.
-        timed_sell(op, op->m1, in, op->d_y, st);
-        timed_sell(op, op->m2, op->d_y, op->d_target, st, op->d_t_pos);
-        timed_sell(op, op->m3, op->d_target, op->d_t2, st, op->d_t_row);
-        timed_sell(op, op->m2t, op->d_t2, op->d_y, st);
+        cyclic_front(op, in, st);
         timed_sell(op, op->m1t, op->d_y, out, st);
         return;
     }
@@ -214,6 +230,22 @@ void pgw::apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipSt
         t.done();
     }
     backward_from_target(op, out, st);
+}
+
+void pgw::apply_and_project(pgw_op* op, const uint32_t* in, uint32_t* out, int mode, uint32_t* seq0,
+                            uint32_t* seq1, const uint32_t* pos, uint32_t pos_off, hipStream_t st) {
+    if (op->kind != PGW_KIND_SYNTHETIC && op->b == 4 && !op->gram_mfma && fused_gram_allowed()) {
+        cyclic_front(op, in, st);
+        if (launch_sell_spmm_gram(op->m1t, op->d_y, out, in, mode, seq0, seq1, op->d_gram,
+                                  gram_scratch_bytes(op->b), pos, pos_off, op->m, st))
+            return;
+        timed_sell(op, op->m1t, op->d_y, out, st);
+    } else {
+        apply_square_impl(op, in, out, st);
+    }
+    const uint32_t* x = mode == 1 ? in : out;
+    const uint32_t* y = mode == 1 ? out : in;
+    launch_gram(x, y, op->cols, op->b, mode, seq0, seq1, op->d_gram, op->m, st, pos, pos_off, op->gram_mfma);
 }
 
 namespace {
@@ -766,10 +798,8 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
             std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
             PGW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                apply_square_impl(op, d_B, d_D, st);
-                launch_gram(d_B, d_D, N, op->b, 1, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 0, op->gram_mfma);
-                apply_square_impl(op, d_D, d_B, st);
-                launch_gram(d_B, d_D, N, op->b, 2, d_seq, d_seq + ut, op->d_gram, op->m, st, d_pos, 2, op->gram_mfma);
+                apply_and_project(op, d_B, d_D, 1, d_seq, d_seq + ut, d_pos, 0, st);
+                apply_and_project(op, d_D, d_B, 2, d_seq, d_seq + ut, d_pos, 2, st);
                 launch_advance_pos(d_pos, 4, st);
             } catch (...) {
                 hipGraph_t g = nullptr;

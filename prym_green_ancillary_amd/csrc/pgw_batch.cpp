@@ -69,12 +69,8 @@ struct BatchJob {
 void enqueue_round(const BatchJob& j, hipStream_t st) {
     pgw_op* op = j.op;
     const uint64_t ut = (uint64_t)op->b * (op->b + 1) / 2;
-    apply_square_impl(op, j.d_B, j.d_D, st);
-    launch_gram(j.d_B, j.d_D, op->cols, op->b, 1, j.d_seq, j.d_seq + ut, op->d_gram, op->m, st,
-                j.d_pos, 0, op->gram_mfma);
-    apply_square_impl(op, j.d_D, j.d_B, st);
-    launch_gram(j.d_B, j.d_D, op->cols, op->b, 2, j.d_seq, j.d_seq + ut, op->d_gram, op->m, st,
-                j.d_pos, 2, op->gram_mfma);
+    apply_and_project(op, j.d_B, j.d_D, 1, j.d_seq, j.d_seq + ut, j.d_pos, 0, st);
+    apply_and_project(op, j.d_D, j.d_B, 2, j.d_seq, j.d_seq + ut, j.d_pos, 2, st);
     launch_advance_pos(j.d_pos, 4, st);
 }
 

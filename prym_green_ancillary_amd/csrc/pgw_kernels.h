@@ -75,6 +75,16 @@ struct SellDevice {
 // rows; a scattered 16-byte store per row instead of a coalesced one).
 void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
                       const ModP& m, hipStream_t st, const uint32_t* out_row = nullptr);
+// b = 4, p < 2^16: out = M in and, in the same launch, the sequence round's
+// two packed projections of out with `other` (the apply's input block):
+// mode 1 writes other^T out to out0 and out^T out to out1, mode 2 out^T other
+// and out^T out (each at row *pos + pos_off when pos is given), through
+// k_gram_finish over the launch's partials in `part`.  Returns false, having
+// launched nothing, when the shape does not qualify (p, alignment, or more
+// partials than part_bytes hold).
+bool launch_sell_spmm_gram(const SellDevice& M, const uint32_t* in, uint32_t* out, const uint32_t* other,
+                           int mode, uint32_t* out0, uint32_t* out1, uint64_t* part, size_t part_bytes,
+                           const uint32_t* pos, uint32_t pos_off, const ModP& m, hipStream_t st);
 
 // elementwise -----------------------------------------------------------------
 void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uint64_t rows,

@@ -194,19 +194,13 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 // SMALLP (p < 2^16): products < 2^32, summed in u64; otherwise products
 // < 2^62 summed as 64-bit low/high word sums.  Exact for rows of any length
 // below 2^32.
+// The sums of one (row, column group) of a sparse map: lo (and hi) as
+// described above, complete in the row's lane 0 after the shuffles.
 template <int B, int CW, int LANES, int U, bool SMALLP, bool PACKED>
-__global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ slice,
-                                                   const uint32_t* __restrict__ ent, uint64_t rows,
-                                                   const uint32_t* __restrict__ in,
-                                                   uint32_t* __restrict__ out, ModP m, uint32_t pack_bits,
-                                                   const uint32_t* __restrict__ out_row) {
-    constexpr int G = B / CW;
-    constexpr int T = G * LANES;  // threads per row; LANES > 1 only when T divides 64
-    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t row = tid / T;
-    const uint32_t g = (uint32_t)((tid % T) / LANES), lane = (uint32_t)(tid % LANES);
-    const bool valid = row < rows;
-    uint64_t lo[CW], hi[CW];
+__device__ __forceinline__ void sell_row_sums(const uint64_t* __restrict__ slice, const uint32_t* __restrict__ ent,
+                                              const uint32_t* __restrict__ in, uint32_t pack_bits, uint64_t row,
+                                              uint32_t g, uint32_t lane, bool valid, uint64_t (&lo)[CW],
+                                              uint64_t (&hi)[CW]) {
 #pragma unroll
     for (int q = 0; q < CW; ++q) lo[q] = hi[q] = 0;
     if (valid) {
@@ -264,11 +258,97 @@ __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ 
                 if (!SMALLP) hi[q] += __shfl_xor(hi[q], off, 64);
             }
     }
+}
+
+template <int B, int CW, int LANES, int U, bool SMALLP, bool PACKED>
+__global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ slice,
+                                                   const uint32_t* __restrict__ ent, uint64_t rows,
+                                                   const uint32_t* __restrict__ in,
+                                                   uint32_t* __restrict__ out, ModP m, uint32_t pack_bits,
+                                                   const uint32_t* __restrict__ out_row) {
+    constexpr int G = B / CW;
+    constexpr int T = G * LANES;  // threads per row; LANES > 1 only when T divides 64
+    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t row = tid / T;
+    const uint32_t g = (uint32_t)((tid % T) / LANES), lane = (uint32_t)(tid % LANES);
+    const bool valid = row < rows;
+    uint64_t lo[CW], hi[CW];
+    sell_row_sums<B, CW, LANES, U, SMALLP, PACKED>(slice, ent, in, pack_bits, row, g, lane, valid, lo, hi);
     if (valid && lane == 0) {
         const uint64_t orow = out_row ? out_row[row] : row;
 #pragma unroll
         for (int q = 0; q < CW; ++q)
             out[orow * B + g * CW + q] = SMALLP ? mod_u64(lo[q], m) : mod_split(hi[q], lo[q], m);
+    }
+}
+
+// The last map of a cyclic apply at b = 4, p < 2^16, with the block
+// projection of the sequence round folded in: besides out = M in it sums,
+// over its rows r, the products of out[r] with `other`[r] (the apply's input
+// block) and with itself, the two packed upper triangles the round pushes:
+// mode 1 (other^T out, out^T out), mode 2 (out^T other, out^T out).  Every
+// product is < 2^32 and is reduced mod p at once, so a wavefront adds 64 of
+// them in 32 bits and a workgroup's four wavefronts stay below 2^24; the
+// workgroup's sums land as k_gram_finish's partials
+// part[(e * nblocks + block) * 2 + {0, 1}] = (sum, 0), e = mat * 10 + pair.
+template <int LANES, int U, bool PACKED>
+__global__ __launch_bounds__(256) void k_sell_spmm_gram(const uint64_t* __restrict__ slice,
+                                                        const uint32_t* __restrict__ ent, uint64_t rows,
+                                                        const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out, ModP m, uint32_t pack_bits,
+                                                        const uint32_t* __restrict__ other, int mode,
+                                                        uint64_t* __restrict__ part) {
+    constexpr int B = 4, P = 10;
+    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t row = tid / LANES;
+    const uint32_t lane = (uint32_t)(tid % LANES);
+    const bool valid = row < rows;
+    const bool mine = valid && lane == 0;
+    uint32_t x[B] = {0, 0, 0, 0};
+    if (mine) {
+        const uint4 v = *reinterpret_cast<const uint4*>(other + row * B);
+        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    }
+    uint64_t lo[B], hi[B];
+    sell_row_sums<B, B, LANES, U, true, PACKED>(slice, ent, in, pack_bits, row, 0, lane, valid, lo, hi);
+    uint32_t o[B] = {0, 0, 0, 0};
+    if (mine) {
+#pragma unroll
+        for (int q = 0; q < B; ++q) o[q] = mod_u64(lo[q], m);
+        *reinterpret_cast<uint4*>(out + row * B) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+    // x mod p for x < 2^32 and p < 2^16: Barrett with floor(2^32 / p)
+    const uint32_t mu = (uint32_t)(m.barrett >> 32);
+    auto red = [&](uint32_t v) {
+        const uint32_t q = __umulhi(v, mu);
+        const uint32_t r = v - q * m.p;
+        return r >= m.p ? r - m.p : r;
+    };
+    uint32_t sums[2 * P];
+    int e = 0;
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = i; j < B; ++j, ++e) {
+            sums[e] = red(mode == 1 ? x[i] * o[j] : o[i] * x[j]);
+            sums[P + e] = red(o[i] * o[j]);
+        }
+    // wavefront sums on the device library's cross-lane (DPP) reduction; every
+    // lane of the workgroup reaches this point
+#pragma unroll
+    for (int k = 0; k < 2 * P; ++k) sums[k] = __ockl_wfred_add_u32(sums[k]);
+    __shared__ uint32_t s_sum[4][2 * P];
+    const int wave = threadIdx.x / 64;
+    if (threadIdx.x % 64 == 0)
+#pragma unroll
+        for (int k = 0; k < 2 * P; ++k) s_sum[wave][k] = sums[k];
+    __syncthreads();
+    if (threadIdx.x < 2 * P) {
+        const uint32_t t = threadIdx.x;
+        const uint64_t v = (uint64_t)s_sum[0][t] + s_sum[1][t] + s_sum[2][t] + s_sum[3][t];
+        uint64_t* o2 = part + ((uint64_t)t * gridDim.x + blockIdx.x) * 2;
+        o2[0] = v;
+        o2[1] = 0;
     }
 }
 
@@ -341,6 +421,46 @@ void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, ui
         default: throw Error(PGW_ERR_UNSUPPORTED, "block width must be in 1..16");
     }
     PGW_CHECK_LAUNCH("sparse map");
+}
+
+__global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict__ part,
+                                                     unsigned nblocks, uint32_t P,
+                                                     uint32_t* __restrict__ out0,
+                                                     uint32_t* __restrict__ out1, ModP m,
+                                                     const uint32_t* __restrict__ pos,
+                                                     uint32_t pos_off);
+
+bool launch_sell_spmm_gram(const SellDevice& M, const uint32_t* in, uint32_t* out, const uint32_t* other,
+                           int mode, uint32_t* out0, uint32_t* out1, uint64_t* part, size_t part_bytes,
+                           const uint32_t* pos, uint32_t pos_off, const ModP& m, hipStream_t st) {
+    constexpr uint32_t P = 10;
+    auto aligned = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+    if (m.p >= 65536u || M.rows == 0 || !aligned(in) || !aligned(out) || !aligned(other) || (mode != 1 && mode != 2))
+        return false;
+    int lanes = 1;
+    const double width = (double)M.padded / M.rows;
+    while (lanes < 8 && (double)M.rows * lanes < (double)(1 << 16) && 2.0 * lanes <= width) lanes *= 2;
+    const uint64_t blocks = (M.rows * lanes + 255) / 256;
+    if (blocks * 2 * P * 2 * sizeof(uint64_t) > part_bytes || blocks >= (1ULL << 31)) return false;
+    const dim3 grid((unsigned)blocks), blk(256);
+#define PGW_SG(L)                                                                                      \
+    do {                                                                                               \
+        if (M.pack_bits)                                                                               \
+            hipLaunchKernelGGL((k_sell_spmm_gram<L, kSellBatch, true>), grid, blk, 0, st, M.slice, M.ent, \
+                               M.rows, in, out, m, M.pack_bits, other, mode, part);                    \
+        else                                                                                           \
+            hipLaunchKernelGGL((k_sell_spmm_gram<L, kSellBatch, false>), grid, blk, 0, st, M.slice, M.ent, \
+                               M.rows, in, out, m, M.pack_bits, other, mode, part);                    \
+    } while (0)
+    if (lanes == 8) PGW_SG(8);
+    else if (lanes == 4) PGW_SG(4);
+    else if (lanes == 2) PGW_SG(2);
+    else PGW_SG(1);
+#undef PGW_SG
+    hipLaunchKernelGGL(k_gram_finish, dim3(2 * P), dim3(256), 0, st, part, (unsigned)blocks, P, out0, out1, m,
+                       pos, pos_off);
+    PGW_CHECK_LAUNCH("sparse map with projection");
+    return true;
 }
 
 // ===========================================================================

@@ -143,6 +143,13 @@ namespace pgw {
 
 // d_out = S d_in on stream st (the block-apply chain, enqueue only).
 void apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipStream_t st);
+// Half a sequence round: out = S in, then the mode's two projections into
+// seq0 / seq1 at row *pos + pos_off (mode 1: in^T out and out^T out; mode 2:
+// out^T in and out^T out), the same values as apply_square_impl followed by
+// launch_gram(x, y, ..., mode) with (x, y) = (in, out) or (out, in).  Cyclic
+// operators at b = 4, p < 2^16 fold the projection into their last map.
+void apply_and_project(pgw_op* op, const uint32_t* in, uint32_t* out, int mode, uint32_t* seq0,
+                       uint32_t* seq1, const uint32_t* pos, uint32_t pos_off, hipStream_t st);
 
 // Times tile configurations cfg_lo..cfg_hi of each of the four MFMA sweeps on
 // the operator's own buffers (reps launches each, HIP events on the

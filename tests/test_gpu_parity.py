@@ -117,10 +117,14 @@ def test_g12_certificate_on_gpu(torch_cuda, ctx, g12, g12_reference):
     assert host(d_out).any()
 
 
-@pytest.mark.parametrize("mix_order", [True, False])
-def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12, mix_order, monkeypatch):
+@pytest.mark.parametrize("mix_order,fused", [(True, True), (False, True), (True, False)])
+def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12, mix_order, fused, monkeypatch):
+    # fused: the round's block projections summed inside the last sparse map
+    # (b = 4, p < 2^16), else by the projection kernels
     if not mix_order:
         monkeypatch.setenv("PGW_CYCLIC_ORDER", "0")
+    if not fused:
+        monkeypatch.setenv("PGW_CYCLIC_FUSED_GRAM", "0")
     p, sector, b = g12["modulus"], 1, 4
     o = oc.cyclic_op(g12, sector)
     dr, dc, v = pg.reference_random_state(1, sector, p, o.rows, o.cols, b)
