@@ -128,9 +128,9 @@ void timed_limb_split(pgw_op* op, const uint32_t* in, const uint32_t* scale, int
 }
 
 void timed_sell(pgw_op* op, const SellDevice& M, const uint32_t* in, uint32_t* out, hipStream_t st,
-                const uint32_t* out_row = nullptr) {
+                const uint32_t* out_row = nullptr, uint32_t* bump = nullptr) {
     StageTimer t(op, 4, sell_bytes(M, op->b) + (out_row ? M.rows * 4.0 : 0.0), st);
-    launch_sell_spmm(M, in, out, op->b, op->m, st, out_row);
+    launch_sell_spmm(M, in, out, op->b, op->m, st, out_row, bump);
     t.done();
 }
 
@@ -199,9 +199,9 @@ namespace {
 // P^T Dr P are folded in.  With the row-mixing order, M2 stores t by
 // position, M3 (kept in that order) gathers from nearby positions and stores
 // its rows back in natural order, where M2^T gathers them.  Leaves M2^T M3 M2
-// M1 in in op->d_y.
-void cyclic_front(pgw_op* op, const uint32_t* in, hipStream_t st) {
-    timed_sell(op, op->m1, in, op->d_y, st);
+// M1 in in op->d_y; with bump, M1's launch also adds 4 to *bump.
+void cyclic_front(pgw_op* op, const uint32_t* in, hipStream_t st, uint32_t* bump = nullptr) {
+    timed_sell(op, op->m1, in, op->d_y, st, nullptr, bump);
     timed_sell(op, op->m2, op->d_y, op->d_target, st, op->d_t_pos);
     timed_sell(op, op->m3, op->d_target, op->d_t2, st, op->d_t_row);
     timed_sell(op, op->m2t, op->d_t2, op->d_y, st);
@@ -212,6 +212,17 @@ void cyclic_front(pgw_op* op, const uint32_t* in, hipStream_t st) {
 bool fused_gram_allowed() {
     const char* knob = std::getenv("PGW_CYCLIC_FUSED_GRAM");
     return !(knob && knob[0] == '0');
+}
+
+// Whether a sequence round of `op` runs as ten sparse maps (the two last
+// with the projections folded in, each half's partials in its own half of
+// the operator's Gram scratch) and one finish for both halves, the position
+// moving on in the round's first map; *nb receives the partial count.
+bool fused_round_plan(const pgw_op* op, unsigned* nb) {
+    if (op->kind == PGW_KIND_SYNTHETIC || op->b != 4 || op->gram_mfma || !fused_gram_allowed() || op->m1.rows == 0)
+        return false;
+    *nb = sell_gram_plan(op->m1t, gram_scratch_bytes(op->b) / 2, op->m);
+    return *nb != 0;
 }
 
 }  // namespace
@@ -786,7 +797,10 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
             PGW_HIP(hipMalloc(&d_D, N * b * sizeof(uint32_t)));
             PGW_HIP(hipMalloc(&d_seq, L * ut * sizeof(uint32_t)));
             PGW_HIP(hipMalloc(&d_pos, sizeof(uint32_t)));
-            const uint32_t one = 1;  // the next sequence position, read on the device
+            // the next sequence position, read on the device; a fused round
+            // moves it on at its start, so it starts one round back
+            unsigned nb_unused = 0;
+            const uint32_t one = fused_round_plan(op, &nb_unused) ? 1u - 4u : 1u;
             PGW_HIP(hipMemcpy(d_pos, &one, sizeof(uint32_t), hipMemcpyHostToDevice));
         }
         PGW_HIP(hipMemcpyAsync(d_B, h_v, N * b * sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -798,9 +812,26 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
             std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
             PGW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                apply_and_project(op, d_B, d_D, 1, d_seq, d_seq + ut, d_pos, 0, st);
-                apply_and_project(op, d_D, d_B, 2, d_seq, d_seq + ut, d_pos, 2, st);
-                launch_advance_pos(d_pos, 4, st);
+                unsigned nb = 0;
+                if (fused_round_plan(op, &nb)) {
+                    uint64_t* part_a = op->d_gram;
+                    uint64_t* part_b = op->d_gram + gram_scratch_bytes(op->b) / 2 / sizeof(uint64_t);
+                    const size_t half = gram_scratch_bytes(op->b) / 2;
+                    unsigned na = 0, nb2 = 0;
+                    cyclic_front(op, d_B, st, d_pos);
+                    require(launch_sell_spmm_gram(op->m1t, op->d_y, d_D, d_B, 1, d_seq, d_seq + ut, part_a, half,
+                                                  d_pos, 0, op->m, st, &na),
+                            "fused projection declined a planned round");
+                    cyclic_front(op, d_D, st);
+                    require(launch_sell_spmm_gram(op->m1t, op->d_y, d_B, d_D, 2, d_seq, d_seq + ut, part_b, half,
+                                                  d_pos, 2, op->m, st, &nb2),
+                            "fused projection declined a planned round");
+                    launch_gram_finish_round(part_a, na, part_b, nb2, d_seq, d_seq + ut, op->m, d_pos, st);
+                } else {
+                    apply_and_project(op, d_B, d_D, 1, d_seq, d_seq + ut, d_pos, 0, st);
+                    apply_and_project(op, d_D, d_B, 2, d_seq, d_seq + ut, d_pos, 2, st);
+                    launch_advance_pos(d_pos, 4, st);
+                }
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(st, &g);

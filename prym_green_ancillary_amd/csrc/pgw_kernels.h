@@ -73,18 +73,30 @@ struct SellDevice {
 // out (M.rows x b) = M in (M.cols x b) mod p; out must not overlap in.  With
 // out_row, row r of M lands at row out_row[r] of out (a permutation of the
 // rows; a scattered 16-byte store per row instead of a coalesced one).
+// With bump, the launch also adds 4 to *bump (the sequence loop's position).
 void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
-                      const ModP& m, hipStream_t st, const uint32_t* out_row = nullptr);
+                      const ModP& m, hipStream_t st, const uint32_t* out_row = nullptr,
+                      uint32_t* bump = nullptr);
 // b = 4, p < 2^16: out = M in and, in the same launch, the sequence round's
 // two packed projections of out with `other` (the apply's input block):
 // mode 1 writes other^T out to out0 and out^T out to out1, mode 2 out^T other
 // and out^T out (each at row *pos + pos_off when pos is given), through
 // k_gram_finish over the launch's partials in `part`.  Returns false, having
 // launched nothing, when the shape does not qualify (p, alignment, or more
-// partials than part_bytes hold).
+// partials than part_bytes hold).  With nblocks, the finish is left to the
+// caller (launch_gram_finish_round) and *nblocks receives the partial count.
 bool launch_sell_spmm_gram(const SellDevice& M, const uint32_t* in, uint32_t* out, const uint32_t* other,
                            int mode, uint32_t* out0, uint32_t* out1, uint64_t* part, size_t part_bytes,
-                           const uint32_t* pos, uint32_t pos_off, const ModP& m, hipStream_t st);
+                           const uint32_t* pos, uint32_t pos_off, const ModP& m, hipStream_t st,
+                           unsigned* nblocks = nullptr);
+// Partial workgroups launch_sell_spmm_gram uses on M with part_bytes of
+// scratch, or 0 when the shape does not qualify (pointer alignment aside).
+unsigned sell_gram_plan(const SellDevice& M, size_t part_bytes, const ModP& m);
+// Both halves of a b = 4 sequence round at once: the first half's partials
+// (part_a, na workgroups; mode 1) to rows *pos and *pos + 1 of out0 / out1,
+// the second's (part_b, nb; mode 2) to rows *pos + 2 and *pos + 3.
+void launch_gram_finish_round(const uint64_t* part_a, unsigned na, const uint64_t* part_b, unsigned nb,
+                              uint32_t* out0, uint32_t* out1, const ModP& m, const uint32_t* pos, hipStream_t st);
 
 // elementwise -----------------------------------------------------------------
 void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uint64_t rows,

@@ -265,13 +265,17 @@ __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ 
                                                    const uint32_t* __restrict__ ent, uint64_t rows,
                                                    const uint32_t* __restrict__ in,
                                                    uint32_t* __restrict__ out, ModP m, uint32_t pack_bits,
-                                                   const uint32_t* __restrict__ out_row) {
+                                                   const uint32_t* __restrict__ out_row,
+                                                   uint32_t* __restrict__ bump) {
     constexpr int G = B / CW;
     constexpr int T = G * LANES;  // threads per row; LANES > 1 only when T divides 64
     const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     const uint64_t row = tid / T;
     const uint32_t g = (uint32_t)((tid % T) / LANES), lane = (uint32_t)(tid % LANES);
     const bool valid = row < rows;
+    // the sequence loop's position counter moves on here, in the round's
+    // first launch, while no other kernel reads it
+    if (bump && tid == 0) *bump += 4;
     uint64_t lo[CW], hi[CW];
     sell_row_sums<B, CW, LANES, U, SMALLP, PACKED>(slice, ent, in, pack_bits, row, g, lane, valid, lo, hi);
     if (valid && lane == 0) {
@@ -357,12 +361,12 @@ constexpr int kSellBatch = 8;
 
 template <int B, int CW, int LANES>
 static void sell_launch(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
-                        hipStream_t st, const uint32_t* out_row) {
+                        hipStream_t st, const uint32_t* out_row, uint32_t* bump) {
     const uint64_t threads = M.rows * (B / CW) * LANES;
     const dim3 grid(grid_for(threads)), blk(256);
 #define PGW_SELL(SP, PK)                                                                                  \
     hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, kSellBatch, SP, PK>), grid, blk, 0, st, M.slice, M.ent,     \
-                       M.rows, in, out, m, M.pack_bits, out_row)
+                       M.rows, in, out, m, M.pack_bits, out_row, bump)
     if (m.p < 65536u) {
         if (M.pack_bits) PGW_SELL(true, true); else PGW_SELL(true, false);
     } else {
@@ -379,7 +383,7 @@ static void sell_launch(const SellDevice& M, const uint32_t* in, uint32_t* out, 
 // threads must stay inside one wavefront)
 template <int B, int CW>
 static void sell_b(const SellDevice& M, const uint32_t* in, uint32_t* out, const ModP& m,
-                   hipStream_t st, const uint32_t* out_row) {
+                   hipStream_t st, const uint32_t* out_row, uint32_t* bump) {
     constexpr int G = B / CW;
     int lanes = 1;
     if ((G & (G - 1)) == 0) {
@@ -388,36 +392,37 @@ static void sell_b(const SellDevice& M, const uint32_t* in, uint32_t* out, const
             lanes *= 2;
     }
     if constexpr ((G & (G - 1)) == 0 && G <= 8) {
-        if (lanes == 8) return sell_launch<B, CW, 8>(M, in, out, m, st, out_row);
-        if (lanes == 4) return sell_launch<B, CW, 4>(M, in, out, m, st, out_row);
-        if (lanes == 2) return sell_launch<B, CW, 2>(M, in, out, m, st, out_row);
+        if (lanes == 8) return sell_launch<B, CW, 8>(M, in, out, m, st, out_row, bump);
+        if (lanes == 4) return sell_launch<B, CW, 4>(M, in, out, m, st, out_row, bump);
+        if (lanes == 2) return sell_launch<B, CW, 2>(M, in, out, m, st, out_row, bump);
     }
-    sell_launch<B, CW, 1>(M, in, out, m, st, out_row);
+    sell_launch<B, CW, 1>(M, in, out, m, st, out_row, bump);
 }
 
 void launch_sell_spmm(const SellDevice& M, const uint32_t* in, uint32_t* out, uint32_t b,
-                      const ModP& m, hipStream_t st, const uint32_t* out_row) {
+                      const ModP& m, hipStream_t st, const uint32_t* out_row, uint32_t* bump) {
+    require(M.rows > 0 || bump == nullptr, "position update on an empty sparse map");
     if (M.rows == 0) return;
     require(M.rows < (1ULL << 40), "sparse map too large");
     // 16-byte gathers need 16-byte aligned rows
     const bool vec = b % 4 == 0 && (reinterpret_cast<uintptr_t>(in) % 16) == 0;
     switch (b) {
-        case 1: sell_b<1, 1>(M, in, out, m, st, out_row); break;
-        case 2: sell_b<2, 2>(M, in, out, m, st, out_row); break;
-        case 3: sell_b<3, 3>(M, in, out, m, st, out_row); break;
-        case 4: if (vec) sell_b<4, 4>(M, in, out, m, st, out_row); else sell_b<4, 1>(M, in, out, m, st, out_row); break;
-        case 5: sell_b<5, 1>(M, in, out, m, st, out_row); break;
-        case 6: sell_b<6, 2>(M, in, out, m, st, out_row); break;
-        case 7: sell_b<7, 1>(M, in, out, m, st, out_row); break;
-        case 8: if (vec) sell_b<8, 4>(M, in, out, m, st, out_row); else sell_b<8, 1>(M, in, out, m, st, out_row); break;
-        case 9: sell_b<9, 3>(M, in, out, m, st, out_row); break;
-        case 10: sell_b<10, 2>(M, in, out, m, st, out_row); break;
-        case 11: sell_b<11, 1>(M, in, out, m, st, out_row); break;
-        case 12: if (vec) sell_b<12, 4>(M, in, out, m, st, out_row); else sell_b<12, 1>(M, in, out, m, st, out_row); break;
-        case 13: sell_b<13, 1>(M, in, out, m, st, out_row); break;
-        case 14: sell_b<14, 2>(M, in, out, m, st, out_row); break;
-        case 15: sell_b<15, 3>(M, in, out, m, st, out_row); break;
-        case 16: if (vec) sell_b<16, 4>(M, in, out, m, st, out_row); else sell_b<16, 1>(M, in, out, m, st, out_row); break;
+        case 1: sell_b<1, 1>(M, in, out, m, st, out_row, bump); break;
+        case 2: sell_b<2, 2>(M, in, out, m, st, out_row, bump); break;
+        case 3: sell_b<3, 3>(M, in, out, m, st, out_row, bump); break;
+        case 4: if (vec) sell_b<4, 4>(M, in, out, m, st, out_row, bump); else sell_b<4, 1>(M, in, out, m, st, out_row, bump); break;
+        case 5: sell_b<5, 1>(M, in, out, m, st, out_row, bump); break;
+        case 6: sell_b<6, 2>(M, in, out, m, st, out_row, bump); break;
+        case 7: sell_b<7, 1>(M, in, out, m, st, out_row, bump); break;
+        case 8: if (vec) sell_b<8, 4>(M, in, out, m, st, out_row, bump); else sell_b<8, 1>(M, in, out, m, st, out_row, bump); break;
+        case 9: sell_b<9, 3>(M, in, out, m, st, out_row, bump); break;
+        case 10: sell_b<10, 2>(M, in, out, m, st, out_row, bump); break;
+        case 11: sell_b<11, 1>(M, in, out, m, st, out_row, bump); break;
+        case 12: if (vec) sell_b<12, 4>(M, in, out, m, st, out_row, bump); else sell_b<12, 1>(M, in, out, m, st, out_row, bump); break;
+        case 13: sell_b<13, 1>(M, in, out, m, st, out_row, bump); break;
+        case 14: sell_b<14, 2>(M, in, out, m, st, out_row, bump); break;
+        case 15: sell_b<15, 3>(M, in, out, m, st, out_row, bump); break;
+        case 16: if (vec) sell_b<16, 4>(M, in, out, m, st, out_row, bump); else sell_b<16, 1>(M, in, out, m, st, out_row, bump); break;
         default: throw Error(PGW_ERR_UNSUPPORTED, "block width must be in 1..16");
     }
     PGW_CHECK_LAUNCH("sparse map");
@@ -430,19 +435,32 @@ __global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict_
                                                      const uint32_t* __restrict__ pos,
                                                      uint32_t pos_off);
 
-bool launch_sell_spmm_gram(const SellDevice& M, const uint32_t* in, uint32_t* out, const uint32_t* other,
-                           int mode, uint32_t* out0, uint32_t* out1, uint64_t* part, size_t part_bytes,
-                           const uint32_t* pos, uint32_t pos_off, const ModP& m, hipStream_t st) {
-    constexpr uint32_t P = 10;
-    auto aligned = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
-    if (m.p >= 65536u || M.rows == 0 || !aligned(in) || !aligned(out) || !aligned(other) || (mode != 1 && mode != 2))
-        return false;
+// lanes per row of k_sell_spmm_gram (the rule of sell_b at G = 1)
+static int sell_gram_lanes(const SellDevice& M) {
     int lanes = 1;
     const double width = (double)M.padded / M.rows;
     while (lanes < 8 && (double)M.rows * lanes < (double)(1 << 16) && 2.0 * lanes <= width) lanes *= 2;
-    const uint64_t blocks = (M.rows * lanes + 255) / 256;
-    if (blocks * 2 * P * 2 * sizeof(uint64_t) > part_bytes || blocks >= (1ULL << 31)) return false;
-    const dim3 grid((unsigned)blocks), blk(256);
+    return lanes;
+}
+
+unsigned sell_gram_plan(const SellDevice& M, size_t part_bytes, const ModP& m) {
+    constexpr uint64_t P = 10;
+    if (m.p >= 65536u || M.rows == 0) return 0;
+    const uint64_t blocks = (M.rows * sell_gram_lanes(M) + 255) / 256;
+    if (blocks * 2 * P * 2 * sizeof(uint64_t) > part_bytes || blocks >= (1ULL << 31)) return 0;
+    return (unsigned)blocks;
+}
+
+bool launch_sell_spmm_gram(const SellDevice& M, const uint32_t* in, uint32_t* out, const uint32_t* other,
+                           int mode, uint32_t* out0, uint32_t* out1, uint64_t* part, size_t part_bytes,
+                           const uint32_t* pos, uint32_t pos_off, const ModP& m, hipStream_t st,
+                           unsigned* nblocks) {
+    constexpr uint32_t P = 10;
+    auto aligned = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+    const unsigned blocks = sell_gram_plan(M, part_bytes, m);
+    if (!blocks || !aligned(in) || !aligned(out) || !aligned(other) || (mode != 1 && mode != 2)) return false;
+    const int lanes = sell_gram_lanes(M);
+    const dim3 grid(blocks), blk(256);
 #define PGW_SG(L)                                                                                      \
     do {                                                                                               \
         if (M.pack_bits)                                                                               \
@@ -457,8 +475,12 @@ bool launch_sell_spmm_gram(const SellDevice& M, const uint32_t* in, uint32_t* ou
     else if (lanes == 2) PGW_SG(2);
     else PGW_SG(1);
 #undef PGW_SG
-    hipLaunchKernelGGL(k_gram_finish, dim3(2 * P), dim3(256), 0, st, part, (unsigned)blocks, P, out0, out1, m,
-                       pos, pos_off);
+    if (nblocks) {
+        *nblocks = (unsigned)blocks;  // the caller finishes the sums
+    } else {
+        hipLaunchKernelGGL(k_gram_finish, dim3(2 * P), dim3(256), 0, st, part, (unsigned)blocks, P, out0, out1,
+                           m, pos, pos_off);
+    }
     PGW_CHECK_LAUNCH("sparse map with projection");
     return true;
 }
@@ -820,6 +842,24 @@ __global__ __launch_bounds__(256) void k_gram_finish(const uint64_t* __restrict_
                                                      const uint32_t* __restrict__ pos,
                                                      uint32_t pos_off) {
     gram_finish_one(part, nblocks, P, out0, out1, m, pos, pos_off, blockIdx.x);
+}
+
+// both halves of a sequence round at b = 4: workgroups 0-19 finish the
+// first half's sums (rows *pos, *pos + 1), 20-39 the second's (*pos + 2, + 3)
+__global__ __launch_bounds__(256) void k_gram_finish_round(const uint64_t* __restrict__ part_a, unsigned na,
+                                                           const uint64_t* __restrict__ part_b, unsigned nb,
+                                                           uint32_t* __restrict__ out0, uint32_t* __restrict__ out1,
+                                                           ModP m, const uint32_t* __restrict__ pos) {
+    constexpr uint32_t P = 10;
+    const bool second = blockIdx.x >= 2 * P;
+    gram_finish_one(second ? part_b : part_a, second ? nb : na, P, out0, out1, m, pos, second ? 2u : 0u,
+                    blockIdx.x % (2 * P));
+}
+
+void launch_gram_finish_round(const uint64_t* part_a, unsigned na, const uint64_t* part_b, unsigned nb,
+                              uint32_t* out0, uint32_t* out1, const ModP& m, const uint32_t* pos, hipStream_t st) {
+    hipLaunchKernelGGL(k_gram_finish_round, dim3(40), dim3(256), 0, st, part_a, na, part_b, nb, out0, out1, m, pos);
+    PGW_CHECK_LAUNCH("round projection finish");
 }
 
 __global__ __launch_bounds__(256) void k_gram_finish_group(const GramJob* __restrict__ jobs,
