@@ -511,7 +511,7 @@ def main():
                     help="large: the N=53.8M sector (default); sector: the mid-size sector; "
                          "batch: 64 uneven sectors; serve: Poisson job stream; paper: the "
                          "reference's g20/g22 instances")
-    ap.add_argument("--genus", type=int, default=22, choices=[20, 22, 24, 26], help="paper: instance")
+    ap.add_argument("--genus", type=int, default=22, choices=[20, 22, 24, 26, 28], help="paper: instance")
     ap.add_argument("--paper-sectors", default="", help="paper: comma-separated sectors (default all)")
     ap.add_argument("--eager-rounds", type=int, default=0,
                     help="paper: time this many eager apply+Gram rounds on the first sector instead")

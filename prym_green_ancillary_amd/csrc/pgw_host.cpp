@@ -18,6 +18,18 @@ namespace pgw {
 static thread_local std::string g_last_error = "no error";
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+const uint64_t* host_binom_table() {
+    static const std::vector<uint64_t> t = [] {
+        std::vector<uint64_t> c(64 * 64, 0);
+        for (int a = 0; a < 64; ++a) {
+            c[a * 64] = 1;
+            for (int b = 1; b <= a; ++b) c[a * 64 + b] = c[(a - 1) * 64 + b - 1] + (b < a ? c[(a - 1) * 64 + b] : 0);
+        }
+        return c;
+    }();
+    return t.data();
+}
+
 uint64_t host_binom(uint32_t n, uint32_t k) {
     if (k > n) return 0;
     if (k > n - k) k = n - k;
@@ -247,31 +259,36 @@ void build_phi_host(PhiHost& h, uint32_t nI, uint32_t nB, uint32_t nA, uint32_t 
     h.row_inc_rank.resize(h.row_subsets * h.row_width);
     h.col_inc_code.resize(h.col_subsets * h.col_width);
     h.col_inc_rank.resize(h.col_subsets * h.col_width);
-    for (uint64_t s = 0; s < h.row_subsets; ++s) {
-        uint64_t mask = unrank_mask(s, (int)ln, (int)degree - 1, C);
-        h.row_subset_w[s] = weight_of(mask);
-        uint32_t q = 0;
-        for (uint32_t li = 0; li < ln; ++li) {
-            if (mask >> li & 1ULL) continue;
-            uint32_t position = (uint32_t)__builtin_popcountll(mask & ((1ULL << li) - 1));
-            h.row_inc_code[s * h.row_width + q] = original[li] | ((position & 1u) << 31);
-            h.row_inc_rank[s * h.row_width + q] =
-                (uint32_t)rank_mask(mask | (1ULL << li), (int)ln, (int)degree, C);
-            ++q;
+    // each subset's entries are its own: the subsets split across host threads
+    parallel_rows(h.row_subsets, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t s = lo; s < hi; ++s) {
+            uint64_t mask = unrank_mask(s, (int)ln, (int)degree - 1, C);
+            h.row_subset_w[s] = weight_of(mask);
+            uint32_t q = 0;
+            for (uint32_t li = 0; li < ln; ++li) {
+                if (mask >> li & 1ULL) continue;
+                uint32_t position = (uint32_t)__builtin_popcountll(mask & ((1ULL << li) - 1));
+                h.row_inc_code[s * h.row_width + q] = original[li] | ((position & 1u) << 31);
+                h.row_inc_rank[s * h.row_width + q] =
+                    (uint32_t)rank_mask(mask | (1ULL << li), (int)ln, (int)degree, C);
+                ++q;
+            }
         }
-    }
-    for (uint64_t s = 0; s < h.col_subsets; ++s) {
-        uint64_t mask = unrank_mask(s, (int)ln, (int)degree, C);
-        h.col_subset_w[s] = weight_of(mask);
-        uint32_t q = 0;
-        for (uint32_t li = 0; li < ln; ++li) {
-            if (!(mask >> li & 1ULL)) continue;
-            h.col_inc_code[s * h.col_width + q] = original[li] | ((q & 1u) << 31);
-            h.col_inc_rank[s * h.col_width + q] =
-                (uint32_t)rank_mask(mask & ~(1ULL << li), (int)ln, (int)degree - 1, C);
-            ++q;
+    });
+    parallel_rows(h.col_subsets, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t s = lo; s < hi; ++s) {
+            uint64_t mask = unrank_mask(s, (int)ln, (int)degree, C);
+            h.col_subset_w[s] = weight_of(mask);
+            uint32_t q = 0;
+            for (uint32_t li = 0; li < ln; ++li) {
+                if (!(mask >> li & 1ULL)) continue;
+                h.col_inc_code[s * h.col_width + q] = original[li] | ((q & 1u) << 31);
+                h.col_inc_rank[s * h.col_width + q] =
+                    (uint32_t)rank_mask(mask & ~(1ULL << li), (int)ln, (int)degree - 1, C);
+                ++q;
+            }
         }
-    }
+    });
     // sector layout
     h.row_off.assign(h.row_subsets + 1, 0);
     h.col_off.assign(h.col_subsets + 1, 0);

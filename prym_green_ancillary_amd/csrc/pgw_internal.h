@@ -8,6 +8,8 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/pgw.h"
@@ -152,11 +154,39 @@ __host__ __device__ inline uint64_t rank_mask(uint64_t mask, int n, int k, const
     return r;
 }
 
+// C(a, b) for 0 <= b <= a < 64 (Pascal's triangle, built once; every entry
+// fits in 64 bits), row-major 64 x 64
+const uint64_t* host_binom_table();
+
 struct HostBinom {
+    const uint64_t* t = host_binom_table();
     uint64_t operator()(int a, int b) const {
-        return (a < 0 || b < 0 || b > a) ? 0ULL : host_binom((uint32_t)a, (uint32_t)b);
+        if (a < 0 || b < 0 || b > a) return 0ULL;
+        return a < 64 ? t[a * 64 + b] : host_binom((uint32_t)a, (uint32_t)b);
     }
 };
+
+inline unsigned host_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, h ? h : 1u));
+}
+
+// f(lo, hi) over [0, n) split across host threads
+template <typename F>
+void parallel_rows(uint64_t n, F&& f) {
+    const unsigned T = n < 4096 ? 1u : host_threads();
+    if (T == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint64_t step = (n + T - 1) / T;
+    for (unsigned q = 0; q < T; ++q) {
+        const uint64_t lo = q * step, hi = std::min(n, lo + step);
+        if (lo < hi) th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    for (auto& t : th) t.join();
+}
 
 }  // namespace pgw
 

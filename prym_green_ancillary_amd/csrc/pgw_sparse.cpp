@@ -28,27 +28,6 @@ namespace {
 
 inline uint32_t mulm(uint64_t a, uint64_t b, uint64_t p) { return (uint32_t)(a * b % p); }
 
-unsigned host_threads() {
-    const unsigned h = std::thread::hardware_concurrency();
-    return std::max(1u, std::min(16u, h ? h : 1u));
-}
-
-// f(lo, hi) over [0, n) split across host threads
-template <typename F>
-void parallel_rows(uint64_t n, F&& f) {
-    const unsigned T = n < 4096 ? 1u : host_threads();
-    if (T == 1) {
-        f(0, n);
-        return;
-    }
-    std::vector<std::thread> th;
-    const uint64_t step = (n + T - 1) / T;
-    for (unsigned q = 0; q < T; ++q) {
-        const uint64_t lo = q * step, hi = std::min(n, lo + step);
-        if (lo < hi) th.emplace_back([&f, lo, hi] { f(lo, hi); });
-    }
-    for (auto& t : th) t.join();
-}
 
 // Rows built independently (row r's entries into a per-row list), then
 // flattened into CSR with duplicate columns summed mod p and zeros dropped.

@@ -1,4 +1,4 @@
-"""The reference's own paper instances (data/g20.json, g22.json, g24.json, g26.json).
+"""The reference's own paper instances (data/g20.json, g22.json, g24.json, g26.json, g28.json).
 
 CPU: the host layout of every eliminated sector reproduces the sector shapes
 that data/cases.json records (and the elimination checks mu_w R = I,
@@ -36,7 +36,7 @@ def _instance(genus):
     return json.loads((GOLDEN / f"g{genus}.json").read_text())
 
 
-@pytest.mark.parametrize("genus,sectors", [(20, None), (22, None), (24, [0, 1]), (26, [0])])
+@pytest.mark.parametrize("genus,sectors", [(20, None), (22, None), (24, [0, 1]), (26, [0]), (28, [0, 1])])
 def test_host_layout_matches_recorded_sector_shapes(genus, sectors):
     inst, case = _instance(genus), _case(genus)
     assert inst["modulus"] == case["prime"] and inst["cyclic"]["order"] == case["order"]
