@@ -23,7 +23,13 @@ uint32_t bits_for(uint64_t v) {  // bits of the largest value below v (at least 
 // A sparse map in its device (sliced-ELL) form, owned by the operator's arena.
 // When a column index and a coefficient (< p) fit in 32 bits together, every
 // entry is one packed u32 (column | coefficient << column bits): half the
-// bytes the map streams per apply.  The paper instances up to g26 qualify.
+// bytes the map streams per apply (the paper instances up to g26).  When they
+// do not, each slice of kSellHeight rows whose columns span less than
+// 2^(32 - bits(p)) stores its entries as packed offsets from its smallest
+// column, the others as pairs (g28: all but 177 of M1's and M1^T's 148 k
+// slices).  PGW_SELL_PACK=0 keeps pairs everywhere and PGW_SELL_PACK=slice
+// forces the per-slice form (test knobs); PGW_SELL_SLICE_BITS narrows the
+// per-slice offsets so that a test meets both kinds of slice.
 SellDevice upload_sell(DeviceArena& A, const CsrHost& csr, uint32_t p) {
     const SellHost h = csr_to_sell(csr);
     SellDevice d;
@@ -31,12 +37,11 @@ SellDevice upload_sell(DeviceArena& A, const CsrHost& csr, uint32_t p) {
     d.cols = h.cols;
     d.nnz = h.nnz;
     d.padded = h.slice.back();
-    d.slice = A.upload(h.slice);
     const uint32_t cb = bits_for(std::max<uint64_t>(h.cols, 2));
-    // PGW_SELL_PACK=0 keeps the pairs (a test knob for the unpacked kernels)
     const char* knob = std::getenv("PGW_SELL_PACK");
-    const bool allow = !(knob && knob[0] == '0');
-    if (allow && cb + bits_for(p) <= 32) {
+    const bool pairs_only = knob && knob[0] == '0';
+    const bool force_slice = knob && std::strcmp(knob, "slice") == 0;
+    if (!pairs_only && !force_slice && cb + bits_for(p) <= 32) {
         std::vector<uint32_t> packed(d.padded);
         for (uint64_t q = 0; q < d.padded; ++q) {
             const uint32_t col = h.ent[2 * q], coef = h.ent[2 * q + 1];
@@ -44,11 +49,65 @@ SellDevice upload_sell(DeviceArena& A, const CsrHost& csr, uint32_t p) {
                     "sparse map entry outside its packed range");
             packed[q] = col | (coef << cb);
         }
+        d.slice = A.upload(h.slice);
         d.ent = A.upload(packed);
+        d.words = d.padded;
+        d.form = 1;
         d.pack_bits = cb;
-    } else {
-        d.ent = A.upload(h.ent);
+        return d;
     }
+    uint32_t ob = 32 - bits_for(p);  // offset bits of a packed slice
+    const char* narrow = std::getenv("PGW_SELL_SLICE_BITS");
+    if (narrow) ob = std::min<uint32_t>(ob, (uint32_t)std::max(0, std::atoi(narrow)));
+    if (pairs_only || (!narrow && ob < 8) || ob < 2) {
+        d.slice = A.upload(h.slice);
+        d.ent = A.upload(h.ent);
+        d.words = 2 * d.padded;
+        return d;
+    }
+    const uint64_t nslices = h.slice.size() - 1;
+    std::vector<uint64_t> wslice(nslices + 1, 0);
+    std::vector<uint32_t> sbase(nslices, 0);
+    for (uint64_t s = 0; s < nslices; ++s) {
+        uint32_t lo = ~0u, hi = 0;
+        for (uint64_t q = h.slice[s]; q < h.slice[s + 1]; ++q)
+            if (h.ent[2 * q + 1] != 0) {  // padding entries carry coefficient 0
+                lo = std::min(lo, h.ent[2 * q]);
+                hi = std::max(hi, h.ent[2 * q]);
+            }
+        const uint64_t n = h.slice[s + 1] - h.slice[s];
+        const bool packed = lo == ~0u || (uint64_t)(hi - lo) < (1ULL << ob);
+        if (packed) sbase[s] = (lo == ~0u ? 0u : lo) | 0x80000000u;
+        // pairs start at an even word, so the kernel reads them as 8-byte loads
+        const uint64_t at = packed ? wslice[s] : (wslice[s] + 1) / 2 * 2;
+        wslice[s] = at;
+        wslice[s + 1] = at + (packed ? n : 2 * n);
+    }
+    std::vector<uint32_t> words(wslice[nslices], 0);
+    parallel_rows(nslices, [&](uint64_t lo_s, uint64_t hi_s) {
+        for (uint64_t s = lo_s; s < hi_s; ++s) {
+            const bool packed = sbase[s] >> 31;
+            const uint32_t base = sbase[s] & 0x7fffffffu;
+            for (uint64_t q = h.slice[s]; q < h.slice[s + 1]; ++q) {
+                const uint64_t i = q - h.slice[s];
+                const uint32_t col = h.ent[2 * q], coef = h.ent[2 * q + 1];
+                if (packed) {
+                    require(coef == 0 || col - base < (1u << ob), "sparse map entry outside its slice range");
+                    words[wslice[s] + i] = coef == 0 ? 0u : (col - base) | (coef << ob);
+                } else {
+                    words[wslice[s] + 2 * i] = col;
+                    words[wslice[s] + 2 * i + 1] = coef;
+                }
+            }
+        }
+    });
+    require(d.cols < (1ULL << 31), "per-slice packing needs fewer than 2^31 columns");
+    d.slice = A.upload(wslice);
+    d.ent = A.upload(words);
+    d.sbase = A.upload(sbase);
+    d.words = words.size();
+    d.form = 2;
+    d.pack_bits = ob;
     return d;
 }
 
@@ -116,7 +175,7 @@ double rowmix_bytes(const pgw_op* op) {
 }
 double sell_bytes(const SellDevice& M, uint32_t b) {
     const uint64_t slices = (M.rows + kSellHeight - 1) / kSellHeight;
-    return (double)M.padded * (M.pack_bits ? 4 : 8) + (slices + 1) * 8.0 + (double)M.cols * b * 4 +
+    return (double)M.words * 4 + (slices + 1) * (M.form == 2 ? 12.0 : 8.0) + (double)M.cols * b * 4 +
            (double)M.rows * b * 4;
 }
 

@@ -63,11 +63,19 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 // Sliced-ELL sparse map on the device (layout: SellHost, pgw_internal.h).
 struct SellDevice {
     uint64_t rows = 0, cols = 0, nnz = 0, padded = 0;  // padded: stored entries
+    uint64_t words = 0;                                 // u32 words of ent
     const uint64_t* slice = nullptr;
-    // pack_bits = 0: (column, coefficient) u32 pairs; otherwise one u32 per
-    // entry, column in the low pack_bits bits and the coefficient above them
-    // (when the column count and p fit together in 32 bits)
+    // form 0: (column, coefficient) u32 pairs, slice[] in entries;
+    // form 1: one u32 per entry, column in the low pack_bits bits and the
+    //   coefficient above them (when the column count and p fit together in
+    //   32 bits), slice[] in entries;
+    // form 2: per slice, u32 words holding the column's offset from the
+    //   slice's base column (sbase[s] = base | 1 << 31) when the slice's
+    //   columns span less than 2^pack_bits, else pairs (sbase[s] = 0);
+    //   slice[] in u32 words, every slice starting at an even word
+    int form = 0;
     const uint32_t* ent = nullptr;
+    const uint32_t* sbase = nullptr;
     uint32_t pack_bits = 0;
 };
 // out (M.rows x b) = M in (M.cols x b) mod p; out must not overlap in.  With

@@ -194,10 +194,33 @@ void launch_dense_plain(const DenseSweep& s, const uint32_t* tab, const uint32_t
 // SMALLP (p < 2^16): products < 2^32, summed in u64; otherwise products
 // < 2^62 summed as 64-bit low/high word sums.  Exact for rows of any length
 // below 2^32.
+// Workgroup b of a launch of nb runs on XCD b % 8 (the hardware deals
+// workgroups to the eight XCDs in turn).  With `contiguous`, blocks are
+// renumbered so that each XCD takes a contiguous range of the rows, whose
+// gathers share input lines in that XCD's L2, instead of every eighth block:
+// a bijection on [0, nb).  It pays while a map's input is within reach of the
+// L2s (g24's maps, 323 k rows: 55.7 against 59.1 us per apply); on larger maps
+// the dealt order, all XCDs reading one region at a time, shares lines through
+// the memory-side cache instead (g26: 211.9 against 214.5 us, g28: 1147
+// against 1182 us), so launches of more than kSellXcdRows rows keep it.
+constexpr uint64_t kSellXcdRows = 1ULL << 19;
+__device__ __forceinline__ uint64_t xcd_block(uint32_t b, uint32_t nb, bool contiguous) {
+    if (!contiguous) return b;
+    const uint32_t x = b % 8, q = nb / 8, r = nb % 8;
+    return (uint64_t)x * q + (x < r ? x : r) + b / 8;
+}
+
 // The sums of one (row, column group) of a sparse map: lo (and hi) as
 // described above, complete in the row's lane 0 after the shuffles.
-template <int B, int CW, int LANES, int U, bool SMALLP, bool PACKED>
+// FORM (SellDevice::form): 0 (column, coefficient) pairs; 1 one u32 per
+// entry, column | coefficient << pack_bits; 2 per slice, either such u32
+// words holding the column's offset from the slice's base column
+// (sbase[s] = base | 1 << 31) or pairs (sbase[s] = 0), slice[s] counting u32
+// words.  A slice is kSellHeight rows, so a wavefront never straddles two and
+// its branch on the slice's form is uniform.
+template <int B, int CW, int LANES, int U, bool SMALLP, int FORM>
 __device__ __forceinline__ void sell_row_sums(const uint64_t* __restrict__ slice, const uint32_t* __restrict__ ent,
+                                              const uint32_t* __restrict__ sbase,
                                               const uint32_t* __restrict__ in, uint32_t pack_bits, uint64_t row,
                                               uint32_t g, uint32_t lane, bool valid, uint64_t (&lo)[CW],
                                               uint64_t (&hi)[CW]) {
@@ -205,20 +228,25 @@ __device__ __forceinline__ void sell_row_sums(const uint64_t* __restrict__ slice
     for (int q = 0; q < CW; ++q) lo[q] = hi[q] = 0;
     if (valid) {
         const uint64_t sl = row / kSellHeight;
-        const uint64_t b0 = slice[sl], width = (slice[sl + 1] - b0) / kSellHeight;
-        const uint64_t e0 = b0 + row % kSellHeight;
-        const uint32_t col_mask = PACKED ? (1u << pack_bits) - 1u : 0u;
+        const uint32_t sb = FORM == 2 ? sbase[sl] : 0u;
+        const bool words = FORM == 1 || (FORM == 2 && (sb >> 31) != 0);  // one u32 per entry
+        const uint32_t base = sb & 0x7fffffffu;
+        const uint64_t b0 = slice[sl];
+        const uint64_t width = (slice[sl + 1] - b0) / kSellHeight / (FORM == 2 && !words ? 2 : 1);
+        const uint64_t e0 = (FORM == 2 ? 0 : b0) + row % kSellHeight;
+        const uint32_t* sent = FORM == 2 ? ent + b0 : ent;  // even word offsets: pairs stay 8-byte aligned
+        const uint32_t col_mask = FORM != 0 ? (1u << pack_bits) - 1u : 0u;
         for (uint64_t k0 = lane; k0 < width; k0 += (uint64_t)LANES * U) {
             uint2 c[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint64_t k = k0 + (uint64_t)u * LANES;
                 const uint64_t e = e0 + (k < width ? k : width - 1) * kSellHeight;
-                if constexpr (PACKED) {
-                    const uint32_t w = ent[e];
-                    c[u] = make_uint2(w & col_mask, w >> pack_bits);
+                if (FORM != 0 && words) {
+                    const uint32_t w = sent[e];
+                    c[u] = make_uint2(base + (w & col_mask), w >> pack_bits);
                 } else {
-                    c[u] = reinterpret_cast<const uint2*>(ent)[e];
+                    c[u] = reinterpret_cast<const uint2*>(sent)[e];
                 }
                 if (k >= width) c[u] = make_uint2(0u, 0u);
             }
@@ -260,16 +288,17 @@ __device__ __forceinline__ void sell_row_sums(const uint64_t* __restrict__ slice
     }
 }
 
-template <int B, int CW, int LANES, int U, bool SMALLP, bool PACKED>
+template <int B, int CW, int LANES, int U, bool SMALLP, int FORM>
 __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ slice,
-                                                   const uint32_t* __restrict__ ent, uint64_t rows,
+                                                   const uint32_t* __restrict__ ent,
+                                                   const uint32_t* __restrict__ sbase, uint64_t rows,
                                                    const uint32_t* __restrict__ in,
                                                    uint32_t* __restrict__ out, ModP m, uint32_t pack_bits,
                                                    const uint32_t* __restrict__ out_row,
                                                    uint32_t* __restrict__ bump) {
     constexpr int G = B / CW;
     constexpr int T = G * LANES;  // threads per row; LANES > 1 only when T divides 64
-    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t tid = xcd_block(blockIdx.x, gridDim.x, rows <= kSellXcdRows) * 256 + threadIdx.x;
     const uint64_t row = tid / T;
     const uint32_t g = (uint32_t)((tid % T) / LANES), lane = (uint32_t)(tid % LANES);
     const bool valid = row < rows;
@@ -277,7 +306,7 @@ __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ 
     // first launch, while no other kernel reads it
     if (bump && tid == 0) *bump += 4;
     uint64_t lo[CW], hi[CW];
-    sell_row_sums<B, CW, LANES, U, SMALLP, PACKED>(slice, ent, in, pack_bits, row, g, lane, valid, lo, hi);
+    sell_row_sums<B, CW, LANES, U, SMALLP, FORM>(slice, ent, sbase, in, pack_bits, row, g, lane, valid, lo, hi);
     if (valid && lane == 0) {
         const uint64_t orow = out_row ? out_row[row] : row;
 #pragma unroll
@@ -295,15 +324,16 @@ __global__ __launch_bounds__(256) void k_sell_spmm(const uint64_t* __restrict__ 
 // them in 32 bits and a workgroup's four wavefronts stay below 2^24; the
 // workgroup's sums land as k_gram_finish's partials
 // part[(e * nblocks + block) * 2 + {0, 1}] = (sum, 0), e = mat * 10 + pair.
-template <int LANES, int U, bool PACKED>
+template <int LANES, int U, int FORM>
 __global__ __launch_bounds__(256) void k_sell_spmm_gram(const uint64_t* __restrict__ slice,
-                                                        const uint32_t* __restrict__ ent, uint64_t rows,
+                                                        const uint32_t* __restrict__ ent,
+                                                        const uint32_t* __restrict__ sbase, uint64_t rows,
                                                         const uint32_t* __restrict__ in,
                                                         uint32_t* __restrict__ out, ModP m, uint32_t pack_bits,
                                                         const uint32_t* __restrict__ other, int mode,
                                                         uint64_t* __restrict__ part) {
     constexpr int B = 4, P = 10;
-    const uint64_t tid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t tid = xcd_block(blockIdx.x, gridDim.x, rows <= kSellXcdRows) * 256 + threadIdx.x;
     const uint64_t row = tid / LANES;
     const uint32_t lane = (uint32_t)(tid % LANES);
     const bool valid = row < rows;
@@ -314,7 +344,7 @@ __global__ __launch_bounds__(256) void k_sell_spmm_gram(const uint64_t* __restri
         x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
     }
     uint64_t lo[B], hi[B];
-    sell_row_sums<B, B, LANES, U, true, PACKED>(slice, ent, in, pack_bits, row, 0, lane, valid, lo, hi);
+    sell_row_sums<B, B, LANES, U, true, FORM>(slice, ent, sbase, in, pack_bits, row, 0, lane, valid, lo, hi);
     uint32_t o[B] = {0, 0, 0, 0};
     if (mine) {
 #pragma unroll
@@ -364,13 +394,13 @@ static void sell_launch(const SellDevice& M, const uint32_t* in, uint32_t* out, 
                         hipStream_t st, const uint32_t* out_row, uint32_t* bump) {
     const uint64_t threads = M.rows * (B / CW) * LANES;
     const dim3 grid(grid_for(threads)), blk(256);
-#define PGW_SELL(SP, PK)                                                                                  \
-    hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, kSellBatch, SP, PK>), grid, blk, 0, st, M.slice, M.ent,     \
-                       M.rows, in, out, m, M.pack_bits, out_row, bump)
+#define PGW_SELL(SP, FORM)                                                                                \
+    hipLaunchKernelGGL((k_sell_spmm<B, CW, LANES, kSellBatch, SP, FORM>), grid, blk, 0, st, M.slice, M.ent,   \
+                       M.sbase, M.rows, in, out, m, M.pack_bits, out_row, bump)
     if (m.p < 65536u) {
-        if (M.pack_bits) PGW_SELL(true, true); else PGW_SELL(true, false);
+        if (M.form == 2) PGW_SELL(true, 2); else if (M.form == 1) PGW_SELL(true, 1); else PGW_SELL(true, 0);
     } else {
-        if (M.pack_bits) PGW_SELL(false, true); else PGW_SELL(false, false);
+        if (M.form == 2) PGW_SELL(false, 2); else if (M.form == 1) PGW_SELL(false, 1); else PGW_SELL(false, 0);
     }
 #undef PGW_SELL
 }
@@ -461,20 +491,19 @@ bool launch_sell_spmm_gram(const SellDevice& M, const uint32_t* in, uint32_t* ou
     if (!blocks || !aligned(in) || !aligned(out) || !aligned(other) || (mode != 1 && mode != 2)) return false;
     const int lanes = sell_gram_lanes(M);
     const dim3 grid(blocks), blk(256);
-#define PGW_SG(L)                                                                                      \
-    do {                                                                                               \
-        if (M.pack_bits)                                                                               \
-            hipLaunchKernelGGL((k_sell_spmm_gram<L, kSellBatch, true>), grid, blk, 0, st, M.slice, M.ent, \
-                               M.rows, in, out, m, M.pack_bits, other, mode, part);                    \
-        else                                                                                           \
-            hipLaunchKernelGGL((k_sell_spmm_gram<L, kSellBatch, false>), grid, blk, 0, st, M.slice, M.ent, \
-                               M.rows, in, out, m, M.pack_bits, other, mode, part);                    \
+#define PGW_SGF(L, FORM)                                                                                 \
+    hipLaunchKernelGGL((k_sell_spmm_gram<L, kSellBatch, FORM>), grid, blk, 0, st, M.slice, M.ent, M.sbase,      \
+                       M.rows, in, out, m, M.pack_bits, other, mode, part)
+#define PGW_SG(L)                                                                                        \
+    do {                                                                                                 \
+        if (M.form == 2) PGW_SGF(L, 2); else if (M.form == 1) PGW_SGF(L, 1); else PGW_SGF(L, 0);          \
     } while (0)
     if (lanes == 8) PGW_SG(8);
     else if (lanes == 4) PGW_SG(4);
     else if (lanes == 2) PGW_SG(2);
     else PGW_SG(1);
 #undef PGW_SG
+#undef PGW_SGF
     if (nblocks) {
         *nblocks = (unsigned)blocks;  // the caller finishes the sums
     } else {

@@ -72,16 +72,27 @@ def gpu_apply(torch, op, x):
 
 # ------------------------------------------------------------ cyclic (g12) --
 
-@pytest.mark.parametrize("b,packed,mix_order", [(4, True, True), (3, True, True), (4, False, True),
-                                                (4, True, False), (5, False, False)])
-def test_g12_apply_square_all_sectors(torch_cuda, ctx, g12, b, packed, mix_order, monkeypatch):
-    # packed: sparse-map entries as one u32 (column | coefficient << column
-    # bits), which g12's sizes allow; PGW_SELL_PACK=0 keeps (column,
-    # coefficient) pairs, the form of instances too large to pack.
+def set_sell_form(monkeypatch, form):
+    # sparse-map entry forms: "packed", one u32 (column | coefficient <<
+    # column bits), which g12's sizes allow; "pairs", (column, coefficient);
+    # "slice", per slice of 64 rows packed offsets from the slice's smallest
+    # column (the form of g28 and larger); "slice4", the same with 4-bit
+    # offsets, so that g12 has slices of both kinds
+    if form == "pairs":
+        monkeypatch.setenv("PGW_SELL_PACK", "0")
+    elif form in ("slice", "slice4"):
+        monkeypatch.setenv("PGW_SELL_PACK", "slice")
+        if form == "slice4":
+            monkeypatch.setenv("PGW_SELL_SLICE_BITS", "4")
+
+
+@pytest.mark.parametrize("b,form,mix_order", [(4, "packed", True), (3, "packed", True), (4, "pairs", True),
+                                              (4, "packed", False), (5, "pairs", False), (4, "slice", True),
+                                              (4, "slice4", True), (3, "slice4", False)])
+def test_g12_apply_square_all_sectors(torch_cuda, ctx, g12, b, form, mix_order, monkeypatch):
     # mix_order: t and M3 = P^T Dr P in the row-mixing order (the default);
     # PGW_CYCLIC_ORDER=0 keeps the natural row order
-    if not packed:
-        monkeypatch.setenv("PGW_SELL_PACK", "0")
+    set_sell_form(monkeypatch, form)
     if not mix_order:
         monkeypatch.setenv("PGW_CYCLIC_ORDER", "0")
     p = g12["modulus"]
@@ -117,10 +128,13 @@ def test_g12_certificate_on_gpu(torch_cuda, ctx, g12, g12_reference):
     assert host(d_out).any()
 
 
-@pytest.mark.parametrize("mix_order,fused", [(True, True), (False, True), (True, False)])
-def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12, mix_order, fused, monkeypatch):
+@pytest.mark.parametrize("mix_order,fused,form", [(True, True, "packed"), (False, True, "packed"),
+                                                 (True, False, "packed"), (True, True, "slice4"),
+                                                 (True, True, "pairs")])
+def test_g12_sequence_and_kernel_vectors(torch_cuda, ctx, g12, mix_order, fused, form, monkeypatch):
     # fused: the round's block projections summed inside the last sparse map
     # (b = 4, p < 2^16), else by the projection kernels
+    set_sell_form(monkeypatch, form)
     if not mix_order:
         monkeypatch.setenv("PGW_CYCLIC_ORDER", "0")
     if not fused:
