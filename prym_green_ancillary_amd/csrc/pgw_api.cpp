@@ -236,15 +236,15 @@ void forward_into_target(pgw_op* op, const uint32_t* x, hipStream_t st) {
     }
 }
 
-// out = Dc F^T (op->d_target), synthetic operator
-void backward_from_target(pgw_op* op, uint32_t* out, hipStream_t st) {
+// out = Dc F^T (target), synthetic operator
+void backward_from_target(pgw_op* op, const uint32_t* target, uint32_t* out, hipStream_t st) {
     if (op->method == PGW_METHOD_MFMA) {
-        timed_limb_split(op, op->d_target, nullptr, op->d_lp_src, op->u_src, op->M, st);
+        timed_limb_split(op, target, nullptr, op->d_lp_src, op->u_src, op->M, st);
         run_sweep(op, 2, nullptr, op->d_lp_src, op->d_mid, nullptr, st);
         timed_limb_split(op, op->d_mid, nullptr, op->d_lp_mid, op->u_tgt, op->Mp, st);
         run_sweep(op, 3, nullptr, op->d_lp_mid, out, op->d_dc, st);
     } else {
-        run_sweep(op, 2, op->d_target, nullptr, op->d_mid, nullptr, st);
+        run_sweep(op, 2, target, nullptr, op->d_mid, nullptr, st);
         run_sweep(op, 3, op->d_mid, nullptr, out, op->d_dc, st);
     }
 }
@@ -293,13 +293,21 @@ void pgw::apply_square_impl(pgw_op* op, const uint32_t* in, uint32_t* out, hipSt
         return;
     }
     forward_into_target(op, in, st);
+    if (op->chains.n) {
+        // P^T Dr P in one pass along the row-mixing chains, into d_target2
+        StageTimer t(op, 2, 2.0 * op->rows * op->b * 4 + op->chains.n * 17.0, st);
+        launch_mix_chains(op->chains, op->d_target, op->d_target2, op->b, op->m, st);
+        t.done();
+        backward_from_target(op, op->d_target2, out, st);
+        return;
+    }
     {
         StageTimer t(op, 2, rowmix_bytes(op), st);
         launch_target_precond(op->d_target, op->d_pairs, op->pair_count, op->rounds, op->d_dr,
                               op->unpaired, op->rows, op->b, op->m, st);
         t.done();
     }
-    backward_from_target(op, out, st);
+    backward_from_target(op, op->d_target, out, st);
 }
 
 void pgw::apply_and_project(pgw_op* op, const uint32_t* in, uint32_t* out, int mode, uint32_t* seq0,
@@ -729,6 +737,20 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             op->d_t2 = A.alloc<uint32_t>(op->rows * b);
         }
         op->d_target = A.alloc<uint32_t>(op->rows * b);
+        // synthetic operators: P^T Dr P in one pass along the chains the one or
+        // two row-mixing rounds link, instead of 2R - 1 passes over the pairs
+        // (PGW_MIX_CHAINS=0 keeps the passes, a test knob)
+        const char* chains_knob = std::getenv("PGW_MIX_CHAINS");
+        if (!cyclic && (op->rounds == 1 || op->rounds == 2) && !(chains_knob && chains_knob[0] == '0')) {
+            MixChains mc = mix_chains(op->rows, op->rounds, h_pairs, h_row_precond);
+            op->chains.n = mc.row.size();
+            op->chains.row = A.upload(mc.row);
+            op->chains.a = A.upload(mc.a);
+            op->chains.b = A.upload(mc.b);
+            op->chains.dr = A.upload(mc.dr);
+            op->chains.flag = A.upload(mc.flag);
+            op->d_target2 = A.alloc<uint32_t>(op->rows * b);
+        }
         op->d_gram = A.alloc<uint64_t>(gram_scratch_bytes(b) / sizeof(uint64_t));
         ctx->live.fetch_add(1);
         *out = op;

@@ -735,6 +735,7 @@ uint64_t estimate_job_bytes(const pgw_operator_desc& d, uint64_t rows, uint64_t 
         bytes += 2 * us * 4 * b * (((uint64_t)d.m_src + 31) / 32 * 32);   // source digit planes
         bytes += 2 * ut2 * 4 * b * (((uint64_t)d.m_tgt + 31) / 32 * 32);  // mid digit planes
         bytes += (us + ut2) * ((uint64_t)d.n + 1) * 4;                    // incidence tables
+        bytes += cols * b * 4 + (cols + (cols / 2 + 1) * 2 * kMixHalo) * 17;  // row-mixing chains
     } else {
         // y', the mixed target and the five sparse maps of the chain (sliced
         // ELL, 8 bytes per entry; rows of the paper instances hold < 100

@@ -315,6 +315,20 @@ void build_cyclic_maps(CyclicMaps& out, const CyclicLayout& L, const pgw_operato
 // partners, and theirs, sit a few positions away).  order[i] = the row at
 // position i.
 std::vector<uint32_t> mix_order(uint64_t rows, uint32_t rounds, const pgw_rowmix_pair* pairs);
+// The target rows as the chains the row-mixing rounds link (rounds <= 2:
+// paths and cycles), for the one-pass P^T Dr P (k_mix_chain): position i holds
+// target row row[i]; flag[i] bits 0-1 give the round of the pair linking
+// positions i and i + 1 (kMixNone: none), bit 2 says row[i] is that pair's
+// `first` (else row[i + 1] is), bit 3 marks a ghost (a copy of a cycle row
+// placed so that every row's neighbours within kMixHalo positions are at
+// hand; it is read, never written); a[i], b[i] are that pair's coefficients
+// and dr[i] = Dr[row[i]].  Every target row appears exactly once unghosted.
+constexpr uint32_t kMixNone = 3, kMixFirst = 4, kMixGhost = 8, kMixHalo = 4;
+struct MixChains {
+    std::vector<uint32_t> row, a, b, dr;
+    std::vector<uint8_t> flag;
+};
+MixChains mix_chains(uint64_t rows, uint32_t rounds, const pgw_rowmix_pair* pairs, const uint32_t* dr);
 // A with rows and columns renumbered: row i of the result is row order[i] of
 // A, column c becomes the position of c in `order` (square A).
 CsrHost permute_square(const CsrHost& A, const std::vector<uint32_t>& order);

@@ -109,6 +109,19 @@ void launch_gram_finish_round(const uint64_t* part_a, unsigned na, const uint64_
 // elementwise -----------------------------------------------------------------
 void launch_scale_rows(const uint32_t* in, const uint32_t* s, uint32_t* out, uint64_t rows,
                        uint32_t b, const ModP& m, hipStream_t st);
+// The row-mixing chains of an operator on the device (MixChains, pgw_internal.h).
+struct MixChainsDevice {
+    uint64_t n = 0;  // positions, ghosts included
+    const uint32_t* row = nullptr;
+    const uint32_t* a = nullptr;
+    const uint32_t* b = nullptr;
+    const uint32_t* dr = nullptr;
+    const uint8_t* flag = nullptr;
+};
+// out = P^T Dr P in for the one or two rounds the chains encode, in one pass
+// (in and out distinct; every row of out written once).
+void launch_mix_chains(const MixChainsDevice& c, const uint32_t* in, uint32_t* out, uint32_t b, const ModP& m,
+                       hipStream_t st);
 // P^T Dr P on a target block in place (P = `rounds` rounds of paired 2x2 maps).
 // `unpaired_row` is the row missing from the last round's pairs (odd row count) or -1.
 void launch_target_precond(uint32_t* block, const pgw_rowmix_pair* pairs, uint64_t pair_count,

@@ -646,6 +646,137 @@ __global__ __launch_bounds__(256) void k_rowmix_group(const MixJob* __restrict__
     }
 }
 
+// One pass of P^T Dr P along the chains the row-mixing rounds link (rows of
+// a path, or of a cycle with kMixHalo ghost copies of its rows on either
+// side; see MixChains).  A workgroup gathers a window of kMixWindow chain
+// positions into LDS, applies P_0, P_1, Dr, P_1^T, P_0^T to the window's
+// pairs in place (the pairs of one round are disjoint, so a stage has no
+// conflicts), and stores the middle kMixWindow - 2 kMixHalo positions that
+// are not ghosts to their rows of `out`.  An error at a window edge (a pair
+// whose partner lies outside) moves inward one position per mixing stage,
+// four stages in all, so kMixHalo = 4 leaves the stored positions exact.
+// Each row is read once and written once, where the three row-mixing passes
+// read and write every row three times.
+constexpr int kMixWindow = 256;
+
+template <int B, bool M31>
+__global__ __launch_bounds__(256) void k_mix_chain(const uint32_t* __restrict__ crow, const uint32_t* __restrict__ ca,
+                                                   const uint32_t* __restrict__ cb, const uint32_t* __restrict__ cdr,
+                                                   const uint8_t* __restrict__ cflag, uint64_t n,
+                                                   const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                   ModP m) {
+    constexpr int CPL = MixShape<B>::CPL, LPP = MixShape<B>::LPP;
+    constexpr int RPP = 256 / LPP;  // rows per pass of the workgroup
+    constexpr int W = kMixWindow, H = (int)kMixHalo, S = W - 2 * H;
+    __shared__ uint32_t s_x[W * B];
+    __shared__ uint32_t s_a[W], s_b[W], s_dr[W];
+    __shared__ uint8_t s_f[W];
+    const int t = threadIdx.x, rg = t / LPP, lane = t % LPP;
+    const int64_t w0 = (int64_t)blockIdx.x * S - H;
+    for (int i = t; i < W; i += 256) {
+        const int64_t g = w0 + i;
+        const bool ok = g >= 0 && (uint64_t)g < n;
+        s_f[i] = ok ? cflag[g] : (uint8_t)kMixNone;
+        s_a[i] = ok ? ca[g] : 0u;
+        s_b[i] = ok ? cb[g] : 0u;
+        s_dr[i] = ok ? cdr[g] : 0u;
+        // a position at the end of the data has no link past it
+        if (ok && (uint64_t)g + 1 == n) s_f[i] = (uint8_t)((s_f[i] & ~3u) | kMixNone);
+    }
+    if (rg < RPP) {
+        for (int i = rg; i < W; i += RPP) {
+            const int64_t g = w0 + i;
+            uint32_t* x = s_x + i * B + lane * CPL;
+            if (g >= 0 && (uint64_t)g < n) {
+                const uint32_t* src = in + (uint64_t)crow[g] * B + lane * CPL;
+                if constexpr (CPL == 4) {
+                    *reinterpret_cast<uint4*>(x) = *reinterpret_cast<const uint4*>(src);
+                } else {
+                    x[0] = src[0];
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) x[c] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t p = m.p;
+    // one mixing stage: round rd, forward [[1,a],[b,1+ab]] or transpose [[1,b],[a,1+ab]] on (first, second)
+    auto stage = [&](uint32_t rd, bool transpose) {
+        if (rg < RPP) {
+            for (int i = rg; i + 1 < W; i += RPP) {
+                const uint8_t f = s_f[i];
+                if ((f & 3u) != rd) continue;
+                const uint32_t a = s_a[i], b = s_b[i];
+                const uint32_t opab = addmod(1u, red63<M31>((uint64_t)a * b, m), p);
+                uint32_t* fi = s_x + ((f & kMixFirst) ? i : i + 1) * B + lane * CPL;
+                uint32_t* se = s_x + ((f & kMixFirst) ? i + 1 : i) * B + lane * CPL;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const uint64_t xc = fi[c], yc = se[c];
+                    if (transpose) {
+                        fi[c] = red63<M31>(xc + (uint64_t)b * yc, m);
+                        se[c] = red63<M31>((uint64_t)a * xc + (uint64_t)opab * yc, m);
+                    } else {
+                        fi[c] = red63<M31>(xc + (uint64_t)a * yc, m);
+                        se[c] = red63<M31>((uint64_t)b * xc + (uint64_t)opab * yc, m);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    };
+    stage(0, false);
+    stage(1, false);
+    if (rg < RPP)
+        for (int i = rg; i < W; i += RPP) {
+            uint32_t* x = s_x + i * B + lane * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) x[c] = red63<M31>((uint64_t)x[c] * s_dr[i], m);
+        }
+    __syncthreads();
+    stage(1, true);
+    stage(0, true);
+    if (rg < RPP)
+        for (int i = H + rg; i < H + S; i += RPP) {
+            const int64_t g = w0 + i;
+            if (g < 0 || (uint64_t)g >= n || (s_f[i] & kMixGhost)) continue;
+            const uint32_t* x = s_x + i * B + lane * CPL;
+            uint32_t* dst = out + (uint64_t)crow[g] * B + lane * CPL;
+            if constexpr (CPL == 4) {
+                *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(x);
+            } else {
+                dst[0] = x[0];
+            }
+        }
+}
+
+template <int B>
+static void mix_chain_b(const MixChainsDevice& c, const uint32_t* in, uint32_t* out, const ModP& m, hipStream_t st) {
+    constexpr int S = kMixWindow - 2 * (int)kMixHalo;
+    const uint64_t blocks = (c.n + S - 1) / S;
+    require(blocks < (1ULL << 31), "row-mixing chains too long for one launch");
+    if (m.p == 0x7fffffffu)
+        hipLaunchKernelGGL((k_mix_chain<B, true>), dim3((unsigned)blocks), dim3(256), 0, st, c.row, c.a, c.b, c.dr,
+                           c.flag, c.n, in, out, m);
+    else
+        hipLaunchKernelGGL((k_mix_chain<B, false>), dim3((unsigned)blocks), dim3(256), 0, st, c.row, c.a, c.b, c.dr,
+                           c.flag, c.n, in, out, m);
+}
+
+void launch_mix_chains(const MixChainsDevice& c, const uint32_t* in, uint32_t* out, uint32_t b, const ModP& m,
+                       hipStream_t st) {
+    if (c.n == 0) return;
+    // 16-byte row accesses need 16-byte aligned rows
+    require(b % 4 != 0 || (reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0),
+            "row-mixing chains need 16-byte aligned blocks");
+#define PGW_CALL(BB) mix_chain_b<BB>(c, in, out, m, st)
+    PGW_SWITCH_B(b, PGW_CALL)
+#undef PGW_CALL
+    PGW_CHECK_LAUNCH("row-mixing chains");
+}
+
 // the row left out of the last round's pairs (odd row count) is fixed by that
 // round's map, so P_last^T Dr P_last only scales it by its Dr entry
 __global__ void k_scale_one_row(uint32_t* __restrict__ block, uint64_t row,

@@ -101,6 +101,10 @@ struct pgw_op {
     uint32_t* d_scaled = nullptr;  // cols x b
     uint32_t* d_mid = nullptr;     // mid x b
     uint32_t* d_target = nullptr;  // rows x b
+    // synthetic operators: the row-mixing chains (n = 0: the pair passes)
+    // and the block P^T Dr P writes
+    pgw::MixChainsDevice chains;
+    uint32_t* d_target2 = nullptr;
     uint64_t* d_gram = nullptr;
     // block projection on int8 MFMA (else on the vector ALUs): by default for
     // operators on the MFMA path, pgw_operator_set_gram to change it

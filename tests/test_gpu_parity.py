@@ -219,6 +219,24 @@ def test_autotune_keeps_results(torch_cuda, ctx):
     assert (gpu_apply(torch_cuda, op, v) == before).all()
 
 
+@pytest.mark.parametrize("rounds,b,p,n,k,M,Mp", [(2, 16, P31, 8, 3, 20, 22), (1, 8, 65521, 8, 3, 20, 22),
+                                                   (2, 3, 1000003, 7, 3, 3, 4), (1, 5, 1009, 7, 3, 3, 4),
+                                                   (3, 4, P31, 8, 3, 20, 22), (2, 4, 65521, 9, 4, 30, 31)])
+def test_row_mixing_chains_equal_pair_passes(torch_cuda, ctx, monkeypatch, rounds, b, p, n, k, M, Mp):
+    # P^T Dr P in one pass along the chains the rounds link (one or two
+    # rounds; three keep the pair passes) == the pair passes == the oracle,
+    # odd row counts (one row left out of every round) included
+    spec, T, E, N, dr, dc, v, pairs = synthetic_setup(n, k, M, Mp, p, b, tseed=7, rounds=rounds)
+    o = oc.synthetic_op(n, k, M, Mp, p, T, E, dr, dc, rounds, pairs.reshape(-1))
+    want = o.apply_square(v, b)
+    _log(f"row-mixing chains rounds={rounds} b={b} p={p} N={N}")
+    op = pg.Operator.synthetic(ctx, spec, b, dr, dc, rounds, pairs, tables=(T, E))
+    assert (gpu_apply(torch_cuda, op, v) == want).all()
+    monkeypatch.setenv("PGW_MIX_CHAINS", "0")
+    op2 = pg.Operator.synthetic(ctx, spec, b, dr, dc, rounds, pairs, tables=(T, E))
+    assert (gpu_apply(torch_cuda, op2, v) == want).all()
+
+
 def test_odd_rows_and_plain_only_block_widths(torch_cuda, ctx):
     # N odd (one row left out of every round's pairs), b = 3 runs on the plain path
     n, k, M, Mp, p, b = 7, 3, 3, 4, 1000003, 3  # N = 35 * 3 = 105
