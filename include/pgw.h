@@ -193,6 +193,15 @@ int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_
  * dense_gemm_TN_chunked3D_offset_u32 (cuda/prym_cuda_helpers.h:107-137).
  * Results are identical for every choice. */
 int pgw_operator_set_gram(pgw_op* op, uint32_t method);
+/* Synthetic operators with one or two row-mixing rounds: build, once, the
+ * chains the rounds link (host work linear in the rows: about 14 s at 53.8 M
+ * rows) so that every later apply runs P^T Dr P in one pass
+ * (k_mix_chain) instead of 2R - 1 passes over the pairs; the results are
+ * the same.  A no-op for other operators and when already prepared.  Call it
+ * while the operator has no work in flight; pgw_sequence_run calls it itself
+ * for loops of 1024 rounds or more.  No counterpart in the reference (its
+ * row mixing is the pair passes, cuda/cyclic_cuda_helpers.h:109-143). */
+int pgw_operator_prepare_row_mixing(pgw_op* op);
 
 /* pgw_gram_upper without an operator and without any allocation: d_out
  * (b(b+1)/2 values) = packed upper triangle of d_x^T d_y mod `prime` for two

@@ -107,7 +107,7 @@ EXPORTED = [
     "pgw_server_enable_rank", "pgw_server_rank", "pgw_wdm_write", "pgw_bench_stages",
     "pgw_gram_upper_noalloc", "pgw_cyclic_apply_cpu", "pgw_deterministic_block",
     "pgw_wdm_stream_open", "pgw_wdm_stream_terms", "pgw_wdm_stream_finish", "pgw_wdm_stream_destroy",
-    "pgw_operator_set_gram", "pgw_batch_terms",
+    "pgw_operator_set_gram", "pgw_batch_terms", "pgw_operator_prepare_row_mixing",
 ]
 
 
@@ -204,6 +204,7 @@ def lib():
         "pgw_wdm_stream_finish": (I, [vp, vp]),
         "pgw_wdm_stream_destroy": (I, [vp]),
         "pgw_operator_set_gram": (I, [vp, u32]),
+        "pgw_operator_prepare_row_mixing": (I, [vp]),
         "pgw_batch_terms": (I, [vp, u64, u64, vp, ctypes.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
@@ -476,6 +477,12 @@ class Operator:
         """Block projections on int8 MFMA (METHOD_MFMA), the vector ALUs
         (METHOD_PLAIN) or the default (METHOD_AUTO) (pgw_operator_set_gram)."""
         _check(lib().pgw_operator_set_gram(self.h, method))
+
+    def prepare_row_mixing(self):
+        """Build the row-mixing chains once, so that every later apply runs
+        P^T Dr P in one pass (pgw_operator_prepare_row_mixing); a no-op for
+        operators it does not apply to."""
+        _check(lib().pgw_operator_prepare_row_mixing(self.h))
 
     def bench_apply(self, d_a: int, d_b: int, iters: int):
         total, sweep = ctypes.c_double(), ctypes.c_double()

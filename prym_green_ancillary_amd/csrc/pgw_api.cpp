@@ -266,6 +266,9 @@ void cyclic_front(pgw_op* op, const uint32_t* in, hipStream_t st, uint32_t* bump
     timed_sell(op, op->m2t, op->d_t2, op->d_y, st);
 }
 
+// sequence loops of at least this many rounds prepare the row-mixing chains
+constexpr uint64_t kPrepareRounds = 1024;
+
 // PGW_CYCLIC_FUSED_GRAM=0 keeps the projection in its own launches (a test
 // knob, read when a round is enqueued or captured)
 bool fused_gram_allowed() {
@@ -737,20 +740,6 @@ int pgw_operator_create(pgw_ctx* ctx, const pgw_operator_desc* desc,
             op->d_t2 = A.alloc<uint32_t>(op->rows * b);
         }
         op->d_target = A.alloc<uint32_t>(op->rows * b);
-        // synthetic operators: P^T Dr P in one pass along the chains the one or
-        // two row-mixing rounds link, instead of 2R - 1 passes over the pairs
-        // (PGW_MIX_CHAINS=0 keeps the passes, a test knob)
-        const char* chains_knob = std::getenv("PGW_MIX_CHAINS");
-        if (!cyclic && (op->rounds == 1 || op->rounds == 2) && !(chains_knob && chains_knob[0] == '0')) {
-            MixChains mc = mix_chains(op->rows, op->rounds, h_pairs, h_row_precond);
-            op->chains.n = mc.row.size();
-            op->chains.row = A.upload(mc.row);
-            op->chains.a = A.upload(mc.a);
-            op->chains.b = A.upload(mc.b);
-            op->chains.dr = A.upload(mc.dr);
-            op->chains.flag = A.upload(mc.flag);
-            op->d_target2 = A.alloc<uint32_t>(op->rows * b);
-        }
         op->d_gram = A.alloc<uint64_t>(gram_scratch_bytes(b) / sizeof(uint64_t));
         ctx->live.fetch_add(1);
         *out = op;
@@ -822,6 +811,37 @@ int pgw_gram_upper(pgw_op* op, const uint32_t* d_x, const uint32_t* d_y, uint32_
     });
 }
 
+int pgw_operator_prepare_row_mixing(pgw_op* op) {
+    return guarded([&] {
+        require(op != nullptr, "null operator");
+        const char* knob = std::getenv("PGW_MIX_CHAINS");  // =0 keeps the pair passes (a test knob)
+        if (op->kind != PGW_KIND_SYNTHETIC || op->chains.n || !(op->rounds == 1 || op->rounds == 2) ||
+            (knob && knob[0] == '0'))
+            return;
+        PGW_HIP(hipSetDevice(op->ctx->device));
+        const uint64_t pc = op->pair_count, total = (uint64_t)op->rounds * pc;
+        std::vector<pgw_rowmix_pair> pairs(total);
+        std::vector<uint32_t> dr(op->rows);
+        {
+            std::lock_guard<std::mutex> cl(op->ctx->capture_mtx);
+            PGW_HIP(hipStreamSynchronize(op->ctx->stream));
+            if (total)
+                PGW_HIP(hipMemcpy(pairs.data(), op->d_pairs, total * sizeof(pgw_rowmix_pair), hipMemcpyDeviceToHost));
+            PGW_HIP(hipMemcpy(dr.data(), op->d_dr, op->rows * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+        MixChains mc = mix_chains(op->rows, op->rounds, pairs.data(), dr.data());
+        pgw::MixChainsDevice c;
+        c.n = mc.row.size();
+        c.row = op->arena.upload(mc.row);
+        c.a = op->arena.upload(mc.a);
+        c.b = op->arena.upload(mc.b);
+        c.dr = op->arena.upload(mc.dr);
+        c.flag = op->arena.upload(mc.flag);
+        op->d_target2 = op->arena.alloc<uint32_t>(op->rows * op->b);
+        op->chains = c;
+    });
+}
+
 int pgw_operator_set_gram(pgw_op* op, uint32_t method) {
     return guarded([&] {
         require(op != nullptr, "null operator");
@@ -887,6 +907,11 @@ int pgw_sequence_run(pgw_op* op, const uint32_t* h_v, uint64_t max_len, uint32_t
         PGW_HIP(hipMemcpyAsync(d_B, h_v, N * b * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         launch_gram(d_B, d_B, N, op->b, 0, d_seq, nullptr, op->d_gram, op->m, st, nullptr, 0, op->gram_mfma);  // V^T V
         PGW_HIP(hipStreamSynchronize(st));
+        // a long loop of a synthetic operator builds its row-mixing chains
+        // first (host work linear in the rows, repaid many times over)
+        if (rounds >= kPrepareRounds && op->kind == PGW_KIND_SYNTHETIC && !op->chains.n) {
+            if (pgw_operator_prepare_row_mixing(op) != PGW_OK) throw Error(PGW_ERR_HIP, pgw_last_error());
+        }
         if (rounds) {
             // one round (cuprym_cyclic.cu:2439-2449) captured once and replayed:
             // D = S B; B^T D, D^T D; B = S D; B^T D, B^T B; pos += 4

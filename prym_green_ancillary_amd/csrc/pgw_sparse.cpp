@@ -296,100 +296,118 @@ std::vector<uint32_t> mix_order(uint64_t rows, uint32_t rounds, const pgw_rowmix
 
 MixChains mix_chains(uint64_t rows, uint32_t rounds, const pgw_rowmix_pair* pairs, const uint32_t* dr) {
     require(rounds >= 1 && rounds <= 2, "row-mixing chains need one or two rounds");
-    require(rows < (1ULL << 32), "row-mixing chains need fewer than 2^32 rows");
+    require(rows < (1ULL << 31), "row-mixing chains need fewer than 2^31 rows");
     const uint64_t pc = rows / 2;
-    // pair index of each row in each round (~0: unpaired)
-    std::vector<uint32_t> pidx((uint64_t)rounds * rows, ~0u);
+    // partner row in each round, or kNone; the high bit of a row's first
+    // word marks it walked, so a step of the walk touches one cache line: the
+    // next row's, which it reads anyway
+    constexpr uint32_t kNone = 0x7fffffffu, kSeen = 0x80000000u;
+    std::vector<uint32_t> nbr(2 * rows, kNone);
     for (uint32_t r = 0; r < rounds; ++r)
-        for (uint64_t q = 0; q < pc; ++q) {
-            const pgw_rowmix_pair& pr = pairs[(uint64_t)r * pc + q];
-            pidx[(uint64_t)r * rows + pr.first] = (uint32_t)q;
-            pidx[(uint64_t)r * rows + pr.second] = (uint32_t)q;
-        }
-    auto partner = [&](uint32_t r, uint32_t v) -> uint64_t {
-        const uint32_t q = pidx[(uint64_t)r * rows + v];
-        if (q == ~0u) return ~0ULL;
-        const pgw_rowmix_pair& pr = pairs[(uint64_t)r * pc + q];
-        return pr.first == v ? pr.second : pr.first;
-    };
-    MixChains c;
-    c.row.reserve(rows + 64);
-    std::vector<char> seen(rows, 0);
+        parallel_rows(pc, [&](uint64_t lo, uint64_t hi) {
+            for (uint64_t q = lo; q < hi; ++q) {
+                const pgw_rowmix_pair& pr = pairs[(uint64_t)r * pc + q];
+                nbr[2 * (uint64_t)pr.first + r] = pr.second;
+                nbr[2 * (uint64_t)pr.second + r] = pr.first;
+            }
+        });
+    auto partner = [&](uint64_t v, uint32_t r) { return nbr[2 * v + r] & ~kSeen; };
+    auto seen = [&](uint64_t v) { return (nbr[2 * v] & kSeen) != 0; };
+    // the walk: chain positions (row, round of the link to the next position,
+    // ghost flag); the pair coefficients are filled in afterwards, in parallel
+    std::vector<uint32_t> row;
+    std::vector<uint8_t> link;
+    row.reserve(rows + 64);
+    link.reserve(rows + 64);
     std::vector<uint32_t> comp;
-    std::vector<uint8_t> link;  // round of the pair between comp[k] and comp[k + 1]
-    auto emit = [&](uint32_t v, uint32_t next, uint32_t rd, bool ghost) {
-        uint8_t f = (uint8_t)(rd == kMixNone ? kMixNone : rd);
-        uint32_t a = 0, b = 0;
-        if (rd != kMixNone) {
-            const pgw_rowmix_pair& pr = pairs[(uint64_t)rd * pc + pidx[(uint64_t)rd * rows + v]];
-            require(pr.first == next || pr.second == next, "row-mixing chain link is not a pair");
-            a = pr.a;
-            b = pr.b;
-            if (pr.first == v) f |= kMixFirst;
-        }
-        if (ghost) f |= kMixGhost;
-        c.row.push_back(v);
-        c.a.push_back(a);
-        c.b.push_back(b);
-        c.dr.push_back(dr[v]);
-        c.flag.push_back(f);
-    };
-    // walks from `start` over unseen partners; returns whether the walk closes a cycle
-    auto walk = [&](uint64_t start) {
+    std::vector<uint8_t> clink;
+    auto walk = [&](uint64_t start) {  // returns the round closing a cycle, or kMixNone
         comp.clear();
-        link.clear();
+        clink.clear();
         uint64_t cur = start;
         for (;;) {
-            seen[cur] = 1;
+            nbr[2 * cur] |= kSeen;
             comp.push_back((uint32_t)cur);
             uint64_t next = ~0ULL;
             uint32_t via = kMixNone;
             for (uint32_t r = 0; r < rounds && next == ~0ULL; ++r) {
-                const uint64_t q = partner(r, (uint32_t)cur);
-                if (q != ~0ULL && !seen[q]) {
+                const uint32_t q = partner(cur, r);
+                if (q != kNone && !seen(q)) {
                     next = q;
                     via = r;
                 }
             }
             if (next == ~0ULL) break;
-            link.push_back((uint8_t)via);
+            clink.push_back((uint8_t)via);
             cur = next;
         }
-        // a cycle: the last row's other partner is the first row
-        uint32_t close = kMixNone;
         if (comp.size() >= 2)
             for (uint32_t r = 0; r < rounds; ++r)
-                if (partner(r, comp.back()) == comp.front() && (link.empty() || r != link.back())) close = r;
-        if (comp.size() == 2 && rounds == 2 && close == kMixNone) {
-            // both rounds pair the same two rows: a cycle of two
-            if (partner(0, comp[0]) == comp[1] && partner(1, comp[0]) == comp[1]) close = 1 - link[0];
-        }
-        return close;
+                if (partner(comp.back(), r) == comp.front() && r != clink.back()) return r;
+        return kMixNone;
     };
     auto emit_component = [&](uint32_t close) {
         const uint64_t L = comp.size();
         if (close == kMixNone) {  // a path
-            for (uint64_t k = 0; k < L; ++k)
-                emit(comp[k], k + 1 < L ? comp[k + 1] : 0u, k + 1 < L ? link[k] : kMixNone, false);
+            for (uint64_t k = 0; k < L; ++k) {
+                row.push_back(comp[k]);
+                link.push_back((uint8_t)(k + 1 < L ? clink[k] : kMixNone));
+            }
             return;
         }
-        // a cycle: edge k joins positions k and k + 1 (mod L), edge L - 1 is the closure
-        auto edge = [&](uint64_t k) -> uint32_t { return k + 1 < L ? link[k] : close; };
+        // a cycle: edge k joins positions k and k + 1 (mod L), edge L - 1 is the
+        // closure; kMixHalo ghost positions on either side
         for (int64_t k = -(int64_t)kMixHalo; k < (int64_t)(L + kMixHalo); ++k) {
             const uint64_t m = (uint64_t)((k % (int64_t)L + (int64_t)L) % (int64_t)L);
             const bool last = k + 1 == (int64_t)(L + kMixHalo);
-            emit(comp[m], comp[(m + 1) % L], last ? kMixNone : edge(m), k < 0 || k >= (int64_t)L);
+            const uint32_t rd = last ? kMixNone : (m + 1 < L ? clink[m] : close);
+            row.push_back(comp[m]);
+            link.push_back((uint8_t)(rd | ((k < 0 || k >= (int64_t)L) ? kMixGhost : 0u)));
         }
     };
     // paths from their ends first, then the cycles
     for (uint64_t v = 0; v < rows; ++v) {
-        if (seen[v]) continue;
+        if (seen(v)) continue;
         bool end = false;
-        for (uint32_t r = 0; r < rounds; ++r) end |= partner(r, (uint32_t)v) == ~0ULL;
+        for (uint32_t r = 0; r < rounds; ++r) end |= partner(v, r) == kNone;
         if (end) emit_component(walk(v));
     }
     for (uint64_t v = 0; v < rows; ++v)
-        if (!seen[v]) emit_component(walk(v));
+        if (!seen(v)) emit_component(walk(v));
+    std::vector<uint32_t>().swap(nbr);
+    // pair index of each row in each round, for the coefficients
+    std::vector<uint32_t> pidx(2 * rows, ~0u);
+    for (uint32_t r = 0; r < rounds; ++r)
+        parallel_rows(pc, [&](uint64_t lo, uint64_t hi) {
+            for (uint64_t q = lo; q < hi; ++q) {
+                const pgw_rowmix_pair& pr = pairs[(uint64_t)r * pc + q];
+                pidx[2 * (uint64_t)pr.first + r] = (uint32_t)q;
+                pidx[2 * (uint64_t)pr.second + r] = (uint32_t)q;
+            }
+        });
+    MixChains c;
+    const uint64_t n = row.size();
+    c.a.assign(n, 0);
+    c.b.assign(n, 0);
+    c.dr.assign(n, 0);
+    c.flag.assign(n, 0);
+    parallel_rows(n, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) {
+            const uint32_t v = row[i], rd = link[i] & 3u;
+            uint8_t f = link[i];
+            if (rd != kMixNone) {
+                const pgw_rowmix_pair& pr = pairs[(uint64_t)rd * pc + pidx[2 * (uint64_t)v + rd]];
+                require(i + 1 < n && (pr.first == row[i + 1] || pr.second == row[i + 1]),
+                        "row-mixing chain link is not a pair");
+                c.a[i] = pr.a;
+                c.b[i] = pr.b;
+                if (pr.first == v) f |= kMixFirst;
+            }
+            c.flag[i] = f;
+            c.dr[i] = dr[v];
+        }
+    });
+    c.row = std::move(row);
     return c;
 }
 

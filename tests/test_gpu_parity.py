@@ -231,9 +231,13 @@ def test_row_mixing_chains_equal_pair_passes(torch_cuda, ctx, monkeypatch, round
     want = o.apply_square(v, b)
     _log(f"row-mixing chains rounds={rounds} b={b} p={p} N={N}")
     op = pg.Operator.synthetic(ctx, spec, b, dr, dc, rounds, pairs, tables=(T, E))
-    assert (gpu_apply(torch_cuda, op, v) == want).all()
+    assert (gpu_apply(torch_cuda, op, v) == want).all()  # pair passes
+    op.prepare_row_mixing()
+    op.prepare_row_mixing()  # a second call changes nothing
+    assert (gpu_apply(torch_cuda, op, v) == want).all()  # chains
     monkeypatch.setenv("PGW_MIX_CHAINS", "0")
     op2 = pg.Operator.synthetic(ctx, spec, b, dr, dc, rounds, pairs, tables=(T, E))
+    op2.prepare_row_mixing()  # kept on the pair passes
     assert (gpu_apply(torch_cuda, op2, v) == want).all()
 
 
@@ -261,6 +265,8 @@ def test_c1_full_sequence_mfma_and_plain_vs_oracle(torch_cuda, ctx):
     for method in (pg.METHOD_MFMA, pg.METHOD_PLAIN):
         _log(f"c1 GPU sequence method={method}")
         op = pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=method, tables=(T, E))
+        if method == pg.METHOD_PLAIN:
+            op.prepare_row_mixing()  # the one-pass row mixing on this run, the pair passes on the other
         seq, curv = op.sequence(v, max_len)
         assert seq.shape == oseq.shape == (pg.sequence_length(max_len), b * (b + 1) // 2)
         assert (seq == oseq).all(), method
@@ -292,6 +298,7 @@ def test_c2_single_sector_apply_mfma_equals_plain_and_oracle(torch_cuda, ctx):
     torch = torch_cuda
     ops = {m: pg.Operator.synthetic(ctx, spec, b, dr, dc, 2, pairs, method=m, tables=(T, E))
            for m in (pg.METHOD_MFMA, pg.METHOD_PLAIN)}
+    ops[pg.METHOD_MFMA].prepare_row_mixing()  # one-pass row mixing against the plain path's pair passes
     rng = np.random.default_rng(2)
     y = rng.integers(0, p, (N, b)).astype(np.uint32)
     outs = {}
