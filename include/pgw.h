@@ -197,9 +197,10 @@ int pgw_operator_set_gram(pgw_op* op, uint32_t method);
  * chains the rounds link (host work linear in the rows: about 14 s at 53.8 M
  * rows) so that every later apply runs P^T Dr P in one pass
  * (k_mix_chain) instead of 2R - 1 passes over the pairs; the results are
- * the same.  A no-op for other operators and when already prepared.  Call it
- * while the operator has no work in flight; pgw_sequence_run calls it itself
- * for loops of 1024 rounds or more.  No counterpart in the reference (its
+ * the same.  A no-op for other operators, for 2^31 - 1 rows or more, and when
+ * already prepared.  Call it while the operator has no work in flight;
+ * pgw_sequence_run calls it itself for loops of 1024 rounds or more (and keeps
+ * the pair passes if it fails).  No counterpart in the reference (its
  * row mixing is the pair passes, cuda/cyclic_cuda_helpers.h:109-143). */
 int pgw_operator_prepare_row_mixing(pgw_op* op);
 
