@@ -19,16 +19,27 @@
  *   - uniform_int (libstdc++ 11 Lemire draw): golden vectors written by g++;
  *   - Gram / sequence: closed form V^T S^t V against dense powers;
  *   - WDM text: the hand-made diag(1,2) file of the BCW CLI test;
- *   - row-mix plan: structure (disjoint pairs, coefficient range) and the
- *     transpose identity only;
+ *   - seeded sector value, realized row-mix seed, row-mix pair lists, plan
+ *     hash, the Dr/Dc/V draws and the deterministic blocks: golden vectors
+ *     written by the reference's own host C++ (cuda/cuprym_cyclic.cu and
+ *     cuda/cyclic_cuda_helpers.h line ranges compiled by oracle/ref_build.sh;
+ *     tests/golden/ref_plan.json, ref_state.json, tests/test_reference_golden.py);
+ *   - eliminated cyclic forward F (Dc x) and P^T Dr P F (Dc x): the reference's
+ *     own CPU forward on every g12 sector and three g20 sectors
+ *     (tests/golden/ref_forward_g12.json, ref_forward_g20.json);
+ *   - orc_sweep_partial (the CPU baseline): its row ranges equal the dense
+ *     D, E, E^T, D^T sweeps;
+ *   - sector ranks: data/cases.json expected_base_ranks for every g20 and g22
+ *     sector and g24 sectors 0 and 1 (through the product CLI, GPU tests);
  *   - deformation variants: the g12 certificate (tests/test_deformation.py):
  *     [S_c; I] in the kernel of [F0_c | -F0_c S_c] with rank 86 - 2 = 84
  *     (augmented_shape / augmented_nullity), rank 84 of [F0_0 without J | Z0]
  *     (completion_rank), transpose identities.
- * Parity unpinned: the exact row-mix pair lists / plan hashes and sidecar
- * hash strings of a real reference run, and the reference binary's own WDM
- * bytes -- no reference output exists to compare with and the CUDA sources
- * cannot be built here.
+ * The reference's CPU transpose half (F^T) calls a __host__ __device__ helper
+ * that needs the CUDA headers and is not built; F^T is pinned by the transpose
+ * identities and by the product's independent host apply.  No reference WDM
+ * file exists to compare bytes with; the WDM text is pinned by the BCW test
+ * file above.
  */
 #ifndef PGW_ORACLE_H
 #define PGW_ORACLE_H
