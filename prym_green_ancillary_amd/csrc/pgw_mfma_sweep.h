@@ -830,10 +830,11 @@ __device__ __forceinline__ void dense_mfma_areg_body(const MfmaSweepArgs& a, uin
     // this lane's 16 bytes of every table fragment of chunk (i, i2, k0): one
     // 32-bit lane offset per chunk, the fragments at uniform steps from it
     // (row tiles 16 chunk rows apart, limbs one limb block apart).  Every load
-    // is in bounds, so none needs a branch: the tail partner i2 may be n, and
-    // table index n is the all-zero table after the digits; a row tile past
-    // R_pad (its MFMAs are skipped) reloads row tile 0, or table rows 0-15 when
-    // the wave has no rows at all.
+    // is in bounds, so no lane needs a branch: the tail partner i2 may be n,
+    // and table index n is the all-zero table after the digits.  A row tile
+    // past the slice (its MFMAs are skipped) loads nothing, a wave-uniform
+    // branch: in c4's transpose sweep (400 rows = 7 + 6 + 6 + 6 tiles of 8
+    // slots) that is 22 % of the fragment loads, in the forward 12.5 %.
     const uint32_t row_a = row_on[0] ? wg_row0 + wr * RT * 16 + cc : (uint32_t)cc;
     auto load_a = [&](v4i (&fa)[RT][NL], int i, int i2, uint32_t k0) {
         const bool tail = k0 == k_full;
@@ -842,7 +843,10 @@ __device__ __forceinline__ void dense_mfma_areg_body(const MfmaSweepArgs& a, uin
         const int t = (tail && kq >= 32) ? i2 : i;
         const uint32_t off = (uint32_t)t * table_stride + (row_a << wshift) + (uint32_t)(tail ? kq & 31 : kq);
         static_for<RT>([&](auto rt) {
-            const uint32_t roff = row_on[rt] ? (uint32_t)decltype(rt)::value * (16u << wshift) : 0u;
+            // wave-uniform: a row tile past the slice loads nothing (its
+            // MFMAs are skipped, so its fragments may hold anything)
+            if (!row_on[rt]) return;
+            const uint32_t roff = (uint32_t)decltype(rt)::value * (16u << wshift);
             static_for<NL>([&](auto j) {
                 fa[rt][j] = *reinterpret_cast<const v4i*>(base + (off + roff + (uint32_t)decltype(j)::value * limb_block_a));
             });
