@@ -242,6 +242,10 @@ PolyMat mul_range(const PolyMat& A, const PolyMat& B, uint64_t lo, uint64_t hi, 
         for (uint32_t j = 0; j < Cn; ++j) nzB[(uint64_t)t * Cn + j] = entry_nonzero(B.at(t, j), lb);
 
     if (std::min(la, lb) <= kSchoolbookLen) {
+        // an output coefficient sums at most S * min(la, lb) products; when
+        // that many (p-1)^2 fit in 64 bits the products are added unreduced
+        const bool raw = (unsigned __int128)(m.p - 1) * (m.p - 1) * S * std::min(la, lb) <
+                         ((unsigned __int128)1 << 64);
         parallel_for((uint64_t)R * Cn, threads, [&](uint64_t e) {
             const uint32_t i = (uint32_t)(e / Cn), j = (uint32_t)(e % Cn);
             std::vector<uint64_t> acc(top - lo, 0);  // sums of residues < 2^31: no overflow
@@ -253,8 +257,16 @@ PolyMat mul_range(const PolyMat& A, const PolyMat& B, uint64_t lo, uint64_t hi, 
                     if (!a[ka]) continue;
                     const uint64_t kb0 = lo > ka ? lo - ka : 0;
                     const uint64_t kb1 = std::min(lb, top - ka);
-                    for (uint64_t kb = kb0; kb < kb1; ++kb)
-                        acc[ka + kb - lo] += mulmod(a[ka], b[kb], m);
+                    if (kb0 >= kb1) continue;
+                    uint64_t* dst = acc.data() + (ka + kb0 - lo);
+                    const uint32_t* src = b + kb0;
+                    const uint64_t cnt = kb1 - kb0;
+                    const uint64_t av = a[ka];
+                    if (raw) {
+                        for (uint64_t q = 0; q < cnt; ++q) dst[q] += av * src[q];
+                    } else {
+                        for (uint64_t q = 0; q < cnt; ++q) dst[q] += mulmod(a[ka], src[q], m);
+                    }
                 }
             }
             uint32_t* out = C.at(i, j);
@@ -357,8 +369,24 @@ PolyMat mul_range(const PolyMat& A, const PolyMat& B, uint64_t lo, uint64_t hi, 
 // dst -= f * src over [from, to)
 void sub_scaled(uint32_t* dst, const uint32_t* src, uint64_t from, uint64_t to, uint32_t f,
                 const ModP& m) {
-    const uint64_t nf = f ? m.p - f : 0;
-    for (uint64_t q = from; q < to; ++q) dst[q] = mod_u64((uint64_t)dst[q] + nf * src[q], m);
+    const uint32_t nf = f ? m.p - f : 0;
+    if (m.p <= 65536u) {
+        // nf * src < 2^32: one 32-bit Barrett step with mu = floor(2^32 / p)
+        // leaves t - qt p in [0, 3p); two conditional subtractions finish it
+        // (vectorisable, no 128-bit products)
+        const uint32_t p = m.p;
+        const uint64_t mu = ((uint64_t)1 << 32) / p;
+        for (uint64_t q = from; q < to; ++q) {
+            const uint32_t t = nf * src[q];
+            uint32_t r = t - (uint32_t)(((uint64_t)t * mu) >> 32) * p;
+            r = r >= p ? r - p : r;
+            r = r >= p ? r - p : r;
+            const uint32_t x = dst[q] + r;  // < 2p <= 2^17
+            dst[q] = x >= p ? x - p : x;
+        }
+        return;
+    }
+    for (uint64_t q = from; q < to; ++q) dst[q] = mod_u64((uint64_t)dst[q] + (uint64_t)nf * src[q], m);
 }
 
 // d elimination steps on the residual G (m x n, coefficients [0, d)): the
@@ -456,7 +484,7 @@ PolyMat mbasis(const PolyMat& G, uint64_t d, std::vector<uint64_t>& delta, const
     return out;
 }
 
-constexpr uint64_t kLeafOrder = 32;  // orders at or below: iterative steps
+constexpr uint64_t kLeafOrder = 16;  // orders at or below: iterative steps
 
 // pm_basis (sigma_basis.rs:37-73): M' for the first d/2 coefficients, the
 // residual x^-(d/2) M' G mod x^(d - d/2), M'' for it, M = M'' M'.
